@@ -1,0 +1,183 @@
+/*
+ * rtm_hip.h -- C ABI of the MI355X (gfx950) 2D acoustic RTM engine.
+ *
+ * The reference has no foreign-function layer: its engines are C++ classes
+ * behind rtm::RTMEngine (include/rtm_engine.hpp:20-83) selected by
+ * rtm::make_engine (include/rtm_factory.hpp:19-23).  This header flattens that
+ * interface into plain C so that any host (the reference's own C++ factory, the
+ * C++ HipRTM wrapper in include/rtm/rtm_hip_engine.hpp, ctypes) can drive the
+ * HIP kernels.  Each entry point names the reference method it replaces.
+ *
+ * Conventions
+ *   - Every array is depth-fastest: interior index ix*nz+iz, as in the
+ *     reference (include/rtm_types.hpp:10-28).  Traces are trace-major,
+ *     traces[ir*nt_stride + it] (include/rtm_types.hpp:54).
+ *   - Host buffers are caller-owned and only read or written during the call.
+ *   - Functions return RTM_HIP_OK (0) or a negative status; the message is
+ *     available from rtm_hip_last_error(handle).  The C++ wrapper turns a
+ *     negative status into std::runtime_error, which is the reference's error
+ *     behaviour (src/cuda/cuda_check.hpp:7-14).
+ *   - A handle owns one HIP stream on one device, plus every table, wavefield,
+ *     snapshot and image buffer of its engine.  Handles are not thread-safe;
+ *     distinct handles may be used from distinct threads.
+ */
+#ifndef RTM_HIP_H
+#define RTM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTM_HIP_OK            0
+#define RTM_HIP_ERR_ARG      -1   /* invalid argument or state           */
+#define RTM_HIP_ERR_DEVICE   -2   /* HIP runtime error (incl. no device) */
+#define RTM_HIP_ERR_MEMORY   -3   /* device or host allocation failed    */
+
+/* Snapshot storage formats (rtm_hip_setup's params->snapshot_format). */
+#define RTM_HIP_SNAP_F32   0      /* the reference's format (src/rtm/forward.cpp:83-88) */
+#define RTM_HIP_SNAP_F16   1      /* IEEE half + one FP32 scale per snapshot            */
+#define RTM_HIP_SNAP_BF16  2      /* bfloat16 + one FP32 scale per snapshot             */
+
+/* rtm::Grid, interior part (include/rtm_types.hpp:16-29). */
+typedef struct {
+    int   nx, nz;
+    float dx, dz;
+    float ox, oz;
+} rtm_hip_grid;
+
+/* rtm::RTMParams (include/rtm_types.hpp:33-40) plus the snapshot format. */
+typedef struct {
+    int   nb;
+    int   order;
+    float sponge_alpha;
+    int   store_interval;
+    float f0;
+    int   verbose;
+    int   snapshot_format;       /* RTM_HIP_SNAP_*; 0 reproduces the reference */
+} rtm_hip_params;
+
+/* rtm::ShotRecord (include/rtm_types.hpp:49-57). */
+typedef struct {
+    float        sx, sz;         /* source position, metres             */
+    int          nrec;
+    const float* rx;             /* receiver positions, metres (nrec)   */
+    const float* rz;
+    int          nt;             /* samples per trace = trace stride    */
+    float        dt;
+    const float* traces;         /* nrec*nt trace-major, or NULL        */
+} rtm_hip_shot;
+
+/* rtm::StageTimes (include/rtm_engine.hpp:8-13), seconds, accumulated since
+ * setup.  imaging stays 0: imaging is fused into the backward step kernel. */
+typedef struct {
+    double io, forward, backward, imaging, total, h2d, kernel, d2h;
+} rtm_hip_times;
+
+typedef struct {
+    int    nx, nz, nb, half, nxe, nze;
+    int    ldz;                  /* device row pitch of extended fields  */
+    int    nt, store_interval, num_snapshots;
+    size_t snapshot_bytes;       /* device bytes of the snapshot store   */
+    size_t device_bytes;         /* all device buffers of this handle    */
+    int    snapshot_format;
+    int    tile_x;               /* step-kernel tile width along x       */
+    char   device_name[128];
+} rtm_hip_info;
+
+/* Sampled step-kernel timing (HIP events around every `every`-th launch on
+ * the handle's stream).  kind 0 = forward step kernel, 1 = backward. */
+typedef struct {
+    long long launches[2];       /* all step launches since reset        */
+    long long sampled[2];        /* launches bracketed by events         */
+    double    sampled_ms[2];     /* summed event time of sampled launches */
+} rtm_hip_kernel_stats;
+
+typedef struct rtm_hip_engine rtm_hip_engine;
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int         rtm_hip_device_count(void);
+
+/* Replaces CUDARTM's constructor + cudaSetDevice(0) (src/cuda/rtm_cuda.cu:30-33):
+ * binds a handle to `device` and creates its stream.  Fails cleanly with
+ * RTM_HIP_ERR_DEVICE when there is no such device. */
+int         rtm_hip_create(int device, rtm_hip_engine** out);
+
+/* Replaces CUDARTM::~CUDARTM (src/cuda/rtm_cuda.cu:124-138). NULL is a no-op. */
+void        rtm_hip_destroy(rtm_hip_engine* h);
+
+/* Message of the last failure on h (h == NULL: of the last failed create or
+ * of a call made with a NULL handle).  Never NULL. */
+const char* rtm_hip_last_error(const rtm_hip_engine* h);
+
+/* Replaces RTMEngine::setup (include/rtm_engine.hpp:25-27; CPUReferenceRTM::setup
+ * src/rtm/rtm_cpu.cpp:9-88; CUDARTM::setup src/cuda/rtm_cuda.cu:30-116).
+ * v: interior velocity nx*nz, m/s.  Builds (v dt)^2, the Cerjan tapers, the
+ * delayed Ricker wavelet and the stencil coefficients on the host with the
+ * reference's float formulas, uploads them, and allocates the wavefields,
+ * the snapshot store (num_snapshots = (nt-1)/store_interval + 1) and the
+ * image/illumination accumulators (zeroed). */
+int         rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* grid, const float* v,
+                          const rtm_hip_params* params, int nt, float dt);
+
+int         rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out);
+
+/* Replaces RTMEngine::forward_propagation (include/rtm_engine.hpp:34-36;
+ * src/rtm/forward.cpp:97-125; src/cuda/propagation_cuda.cu:160-230).
+ * Propagates the source wavefield and fills the device snapshot store.
+ * snapshots_out: NULL (device-resident, as the CUDA engine) or a host buffer
+ * of num_snapshots*nx*nz floats that receives the stored snapshots
+ * (dequantised when the format is F16/BF16).  recorded: NULL or nrec*nt
+ * floats receiving the modelled traces, trace-major with stride nt. */
+int         rtm_hip_forward(rtm_hip_engine* h, const rtm_hip_shot* shot,
+                            float* snapshots_out, float* recorded);
+
+/* Replaces RTMEngine::backward_propagation (include/rtm_engine.hpp:39-42;
+ * src/rtm/backward.cpp:40-75): back-propagates shot->traces (stride
+ * shot->nt, which may exceed the engine's nt, as with the reference's --nt)
+ * and accumulates I += p_s*p_r, S += p_s^2 into the device image on every
+ * stored step, using the snapshots of the preceding rtm_hip_forward. */
+int         rtm_hip_backward(rtm_hip_engine* h, const rtm_hip_shot* shot);
+
+/* Zero / download the device image and illumination (nx*nz floats each).
+ * Replaces the device image handling of CUDARTM::migrate
+ * (src/cuda/rtm_cuda.cu:222-244). */
+int         rtm_hip_image_zero(rtm_hip_engine* h);
+int         rtm_hip_image_get(rtm_hip_engine* h, float* image, float* illum);
+
+/* Replaces RTMEngine::migrate (include/rtm_engine.hpp:45-47; src/rtm/rtm.cpp:83-103;
+ * src/cuda/rtm_cuda.cu:214-250): zero the image, forward + backward every shot
+ * in order (FP32 running sum on the device), download image and illum. */
+int         rtm_hip_migrate(rtm_hip_engine* h, int nshots, const rtm_hip_shot* shots,
+                            float* image, float* illum);
+
+/* Replaces RTMEngine::model_shot (include/rtm_engine.hpp:50; src/rtm/rtm.cpp:105-111):
+ * forward modelling with trace recording; traces_out nrec*nt (engine nt). */
+int         rtm_hip_model_shot(rtm_hip_engine* h, const rtm_hip_shot* shot, float* traces_out);
+
+/* Replaces RTMEngine::times (include/rtm_engine.hpp:75). */
+int         rtm_hip_get_times(const rtm_hip_engine* h, rtm_hip_times* out);
+int         rtm_hip_reset_times(rtm_hip_engine* h);
+
+/* Step-kernel timing for the roofline figure; every = 0 turns sampling off.
+ * No reference counterpart (the reference never filled times.kernel,
+ * include/rtm_engine.hpp:11). */
+int         rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every);
+int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out);
+
+/* Step-kernel tile width along x (cells per 256-thread workgroup: 4, 8 or 16);
+ * 0 restores the built-in choice.  Takes effect at the next forward/backward.
+ * No reference counterpart (the CUDA block shape is fixed,
+ * src/cuda/propagation_cuda.cu:96-97). */
+int         rtm_hip_set_tile(rtm_hip_engine* h, int tile_x);
+
+/* Replaces rtm::mute_direct_wave (include/rtm_io.hpp:9; src/io/seismic_io.cpp:18-35),
+ * host-only: traces (nrec*nt, stride shot->nt) muted in place. */
+void        rtm_hip_mute_direct_wave(const rtm_hip_shot* shot, float* traces,
+                                     float v_surface, float t_pad, float t_taper);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTM_HIP_H */

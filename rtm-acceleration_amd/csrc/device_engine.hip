@@ -1,0 +1,697 @@
+// device_engine.hip -- the C ABI of include/rtm_hip.h: one handle = one shot
+// engine on one device, everything resident in HBM.
+//
+// Device layout (per handle):
+//   vdt2, three wavefields      [nxe][ldz] float, ldz = nze rounded up to 64
+//                               (256-byte rows; pad cells stay 0)
+//   wx [nxe], wz [ldz]          separable Cerjan tapers
+//   wavelet [nt]                Ricker source
+//   snapshot store              num_snapshots x [nx][nz] float (the reference's
+//                               interior layout, src/rtm/forward.cpp:83-88)
+//   image, illumination         [nx][nz] float, running sums over shots
+//   traces                      [nrec][nt_stride] float, uploaded per shot
+//   per-tile entry tables       injection / recording cells, built per shot
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host_tables.hpp"
+#include "rtm/benchmark.hpp"
+#include "rtm/rtm_types.hpp"
+#include "rtm_hip.h"
+#include "step_kernels.hpp"
+
+using rtm_hip::Dir;
+using rtm_hip::Entry;
+
+namespace {
+
+thread_local std::string g_null_handle_error;
+
+struct EventPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct rtm_hip_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool ready = false;
+
+    // geometry / parameters
+    int nx = 0, nz = 0, nb = 0, half = 0, order = 0, nxe = 0, nze = 0, ldz = 0;
+    int nt = 0, store = 1, nsnap = 0, snap_fmt = 0, verbose = 0;
+    float dx = 0, dz = 0, ox = 0, oz = 0, dt = 0;
+    size_t n_int = 0, n_fld = 0;
+    rtm_hip::Coeffs coef{};
+    int tile_x_user = 0;
+
+    // device buffers
+    float* d_vdt2 = nullptr;
+    float* d_wx = nullptr;
+    float* d_wz = nullptr;
+    float* d_wavelet = nullptr;
+    float* d_fld[3] = {nullptr, nullptr, nullptr};
+    float* d_snap = nullptr;
+    size_t snap_bytes = 0;
+    float* d_image = nullptr;
+    float* d_illum = nullptr;
+    int* d_tile_ptr[2] = {nullptr, nullptr};
+    size_t tile_ptr_cap[2] = {0, 0};
+    Entry* d_ent[2] = {nullptr, nullptr};
+    size_t ent_cap[2] = {0, 0};
+    float* d_traces = nullptr;
+    size_t traces_cap = 0;
+    float* d_rec = nullptr;
+    size_t rec_cap = 0;
+    size_t device_bytes = 0;
+
+    // host staging kept alive until the stream has consumed it
+    std::vector<int> h_tile_ptr[2];
+    std::vector<Entry> h_ent[2];
+
+    // accounting
+    rtm_hip_times times{};
+    int timing_every = 0;
+    long long launches[2] = {0, 0};
+    long long sampled[2] = {0, 0};
+    double sampled_ms[2] = {0, 0};
+    std::vector<EventPair> ev_pool;
+    std::vector<std::pair<int, EventPair>> ev_pending;
+    EventPair loop_ev;
+
+    ~rtm_hip_engine() { release(); }
+
+    int fail(int code, const char* fmt, ...) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+
+    int hip_fail(hipError_t e, const char* what) {
+        return fail(e == hipErrorOutOfMemory ? RTM_HIP_ERR_MEMORY : RTM_HIP_ERR_DEVICE,
+                    "%s: %s (%s)", what, hipGetErrorString(e), hipGetErrorName(e));
+    }
+
+    template <typename T>
+    int alloc(T** p, size_t count, const char* what) {
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return hip_fail(e, what);
+        }
+        device_bytes += count * sizeof(T);
+        return RTM_HIP_OK;
+    }
+
+    template <typename T>
+    int grow(T** p, size_t* cap, size_t count, const char* what) {
+        if (count <= *cap && *p) return RTM_HIP_OK;
+        if (*p) {
+            (void)hipFree(*p);
+            device_bytes -= *cap * sizeof(T);
+            *p = nullptr;
+            *cap = 0;
+        }
+        const size_t n = std::max<size_t>(count, 1);
+        if (int rc = alloc(p, n, what)) return rc;
+        *cap = n;
+        return RTM_HIP_OK;
+    }
+
+    void release() {
+        if (device >= 0) (void)hipSetDevice(device);
+        auto f = [](auto*& p) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+        };
+        f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
+        for (auto& p : d_fld) f(p);
+        f(d_snap); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        for (int k = 0; k < 2; ++k) {
+            f(d_tile_ptr[k]); f(d_ent[k]);
+            tile_ptr_cap[k] = ent_cap[k] = 0;
+        }
+        traces_cap = rec_cap = 0;
+        device_bytes = 0;
+        for (auto& ev : ev_pool) {
+            (void)hipEventDestroy(ev.start);
+            (void)hipEventDestroy(ev.stop);
+        }
+        ev_pool.clear();
+        for (auto& pe : ev_pending) {
+            (void)hipEventDestroy(pe.second.start);
+            (void)hipEventDestroy(pe.second.stop);
+        }
+        ev_pending.clear();
+        if (loop_ev.start) (void)hipEventDestroy(loop_ev.start);
+        if (loop_ev.stop) (void)hipEventDestroy(loop_ev.stop);
+        loop_ev = EventPair{};
+        ready = false;
+    }
+
+    int tile_x() const {
+        if (tile_x_user) return tile_x_user;
+        // Enough workgroups to fill 256 CUs several times over, wider tiles
+        // (less halo) once the grid is large.
+        const long long cells = (long long)nxe * ldz;
+        return cells >= 4LL * 1024 * 1024 ? 16 : (cells >= 512LL * 1024 ? 8 : 4);
+    }
+};
+
+namespace {
+
+// Nearest grid point with clamping to the interior (src/rtm/rtm_cpu.cpp:99-123).
+struct CellIndex {
+    int ixe, ize;
+};
+
+CellIndex to_cell(const rtm_hip_engine* h, float x, float z, int* nclamped) {
+    const int ix0 = static_cast<int>(std::lround((x - h->ox) / h->dx));
+    const int iz0 = static_cast<int>(std::lround((z - h->oz) / h->dz));
+    const int ix = std::clamp(ix0, 0, h->nx - 1);
+    const int iz = std::clamp(iz0, 0, h->nz - 1);
+    if (ix != ix0 || iz != iz0) ++*nclamped;
+    return {ix + h->nb, iz + h->nb};
+}
+
+// Per-tile entry table for one direction.  Entries are ordered by tile, then
+// cell, then code, so at a cell the source (code -1) precedes recordings and
+// duplicate receivers keep ascending receiver order (src/rtm/backward.cpp:23-26).
+int build_entries(rtm_hip_engine* h, int k, const rtm_hip_shot* s, bool with_source,
+                  bool with_receivers, int tile_x) {
+    const int ntz = rtm_hip::step_tiles_z(h->ldz);
+    const int ntx = rtm_hip::step_tiles_x(h->nxe, tile_x);
+    const int ntiles = ntz * ntx;
+    int nclamped = 0;
+    struct Tagged {
+        int tile, cell, code;
+    };
+    std::vector<Tagged> v;
+    v.reserve(static_cast<size_t>(s->nrec) + 1);
+    auto add = [&](CellIndex c, int code) {
+        const int tile = (c.ixe / tile_x) * ntz + c.ize / rtm_hip::kTileZ;
+        v.push_back({tile, c.ixe * h->ldz + c.ize, code});
+    };
+    const CellIndex src = to_cell(h, s->sx, s->sz, &nclamped);
+    if (with_source) add(src, -1);
+    if (with_receivers)
+        for (int ir = 0; ir < s->nrec; ++ir) add(to_cell(h, s->rx[ir], s->rz[ir], &nclamped), ir);
+    else
+        for (int ir = 0; ir < s->nrec; ++ir) (void)to_cell(h, s->rx[ir], s->rz[ir], &nclamped);
+    std::sort(v.begin(), v.end(), [](const Tagged& a, const Tagged& b) {
+        if (a.tile != b.tile) return a.tile < b.tile;
+        if (a.cell != b.cell) return a.cell < b.cell;
+        return a.code < b.code;
+    });
+    if (nclamped > 0 && h->verbose && k == 0)
+        std::fprintf(stderr,
+                     "WARNING: %d of %d source/receiver positions fell outside the "
+                     "%.1f x %.1f m model and were clamped to the edge.\n",
+                     nclamped, s->nrec + 1, (h->nx - 1) * h->dx, (h->nz - 1) * h->dz);
+
+    auto& ptr = h->h_tile_ptr[k];
+    auto& ent = h->h_ent[k];
+    ptr.assign(static_cast<size_t>(ntiles) + 1, 0);
+    ent.resize(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+        ent[i] = Entry{v[i].cell, v[i].code};
+        ++ptr[static_cast<size_t>(v[i].tile) + 1];
+    }
+    for (int t = 0; t < ntiles; ++t) ptr[t + 1] += ptr[t];
+
+    if (int rc = h->grow(&h->d_tile_ptr[k], &h->tile_ptr_cap[k], ptr.size(), "tile table"))
+        return rc;
+    if (int rc = h->grow(&h->d_ent[k], &h->ent_cap[k], std::max<size_t>(ent.size(), 1), "entry table"))
+        return rc;
+    rtm::Timer t;
+    hipError_t e = hipMemcpyAsync(h->d_tile_ptr[k], ptr.data(), ptr.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && !ent.empty())
+        e = hipMemcpyAsync(h->d_ent[k], ent.data(), ent.size() * sizeof(Entry),
+                           hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "upload of injection tables");
+    h->times.h2d += t.elapsed();
+    return RTM_HIP_OK;
+}
+
+EventPair take_event(rtm_hip_engine* h) {
+    if (!h->ev_pool.empty()) {
+        EventPair p = h->ev_pool.back();
+        h->ev_pool.pop_back();
+        return p;
+    }
+    EventPair p;
+    (void)hipEventCreate(&p.start);
+    (void)hipEventCreate(&p.stop);
+    return p;
+}
+
+// Reads back the sampled kernel events once the stream is idle.
+void collect_events(rtm_hip_engine* h) {
+    for (auto& pe : h->ev_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pe.second.start, pe.second.stop) == hipSuccess) {
+            h->sampled_ms[pe.first] += ms;
+            h->sampled[pe.first] += 1;
+        }
+        h->ev_pool.push_back(pe.second);
+    }
+    h->ev_pending.clear();
+}
+
+int check_handle(const rtm_hip_engine* h, bool need_setup) {
+    if (!h) {
+        g_null_handle_error = "null rtm_hip_engine handle";
+        return RTM_HIP_ERR_ARG;
+    }
+    if (need_setup && !h->ready) {
+        const_cast<rtm_hip_engine*>(h)->err = "engine is not set up (call rtm_hip_setup first)";
+        return RTM_HIP_ERR_ARG;
+    }
+    return RTM_HIP_OK;
+}
+
+int validate_shot(rtm_hip_engine* h, const rtm_hip_shot* s, bool need_traces) {
+    if (!s) return h->fail(RTM_HIP_ERR_ARG, "null shot");
+    if (s->nrec < 0) return h->fail(RTM_HIP_ERR_ARG, "negative receiver count");
+    if (s->nrec > 0 && (!s->rx || !s->rz)) return h->fail(RTM_HIP_ERR_ARG, "null receiver coordinates");
+    if (need_traces) {
+        if (s->nrec > 0 && !s->traces) return h->fail(RTM_HIP_ERR_ARG, "backward propagation needs traces");
+        if (s->nt < h->nt)
+            return h->fail(RTM_HIP_ERR_ARG, "shot has %d samples per trace, fewer than the engine's nt=%d",
+                           s->nt, h->nt);
+    }
+    return RTM_HIP_OK;
+}
+
+// Runs one propagation (forward or backward) of h->nt steps.
+int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record, int tile_x) {
+    const int k = dir == Dir::Forward ? 0 : 1;
+    rtm_hip::StepArgs a{};
+    a.vdt2 = h->d_vdt2;
+    a.wx = h->d_wx;
+    a.wz = h->d_wz;
+    a.coef = h->coef;
+    a.nxe = h->nxe; a.nze = h->nze; a.ldz = h->ldz; a.nb = h->nb; a.nx = h->nx; a.nz = h->nz;
+    a.ntz = rtm_hip::step_tiles_z(h->ldz);
+    a.ntx = rtm_hip::step_tiles_x(h->nxe, tile_x);
+    a.ntiles = a.ntz * a.ntx;
+    a.tile_ptr = h->d_tile_ptr[k];
+    a.entries = h->d_ent[k];
+    a.wavelet = h->d_wavelet;
+    a.traces = h->d_traces;
+    a.trace_stride = s->nt;
+    a.rec_out = record ? h->d_rec : nullptr;
+    a.rec_stride = h->nt;
+    a.image = h->d_image;
+    a.illum = h->d_illum;
+
+    for (auto& p : h->d_fld) {
+        hipError_t e = hipMemsetAsync(p, 0, h->n_fld * sizeof(float), h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "wavefield reset");
+    }
+    if (!h->loop_ev.start) {
+        (void)hipEventCreate(&h->loop_ev.start);
+        (void)hipEventCreate(&h->loop_ev.stop);
+    }
+    (void)hipEventRecord(h->loop_ev.start, h->stream);
+
+    int ia = 0, ib = 1, ic = 2;   // p_prev, p_cur, p_next
+    for (int step = 0; step < h->nt; ++step) {
+        const int it = dir == Dir::Forward ? step : h->nt - 1 - step;
+        a.it = it;
+        a.p_prev = h->d_fld[ia];
+        a.p_cur = h->d_fld[ib];
+        a.p_next = h->d_fld[ic];
+        float* slot = (it % h->store) == 0 ? h->d_snap + (size_t)(it / h->store) * h->n_int : nullptr;
+        a.snap_out = dir == Dir::Forward ? slot : nullptr;
+        a.snap_in = dir == Dir::Backward ? slot : nullptr;
+
+        const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
+        EventPair ev;
+        if (sample) {
+            ev = take_event(h);
+            (void)hipEventRecord(ev.start, h->stream);
+        }
+        hipError_t e = rtm_hip::launch_step(dir, h->half, tile_x, a, h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
+        if (sample) {
+            (void)hipEventRecord(ev.stop, h->stream);
+            h->ev_pending.push_back({k, ev});
+        }
+        h->launches[k] += 1;
+        // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
+        const int old = ia;
+        ia = ib;
+        ib = ic;
+        ic = old;
+    }
+    (void)hipEventRecord(h->loop_ev.stop, h->stream);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "step loop");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h->loop_ev.start, h->loop_ev.stop) == hipSuccess)
+        h->times.kernel += 1e-3 * ms;
+    collect_events(h);
+    return RTM_HIP_OK;
+}
+
+int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out, float* recorded) {
+    if (int rc = validate_shot(h, s, false)) return rc;
+    rtm::Timer wall;
+    const int tile_x = h->tile_x();
+    const bool record = recorded != nullptr && s->nrec > 0;
+    if (int rc = build_entries(h, 0, s, true, record, tile_x)) return rc;
+    if (record) {
+        const size_t n = (size_t)s->nrec * h->nt;
+        if (int rc = h->grow(&h->d_rec, &h->rec_cap, n, "recorded traces")) return rc;
+        hipError_t e = hipMemsetAsync(h->d_rec, 0, n * sizeof(float), h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "recorded traces reset");
+    }
+    if (int rc = run_loop(h, Dir::Forward, s, record, tile_x)) return rc;
+    h->times.forward += wall.elapsed();
+
+    rtm::Timer d2h;
+    if (record) {
+        hipError_t e = hipMemcpy(recorded, h->d_rec, (size_t)s->nrec * h->nt * sizeof(float),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return h->hip_fail(e, "download of recorded traces");
+    }
+    if (snapshots_out) {
+        hipError_t e = hipMemcpy(snapshots_out, h->d_snap, (size_t)h->nsnap * h->n_int * sizeof(float),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return h->hip_fail(e, "download of snapshots");
+    }
+    h->times.d2h += d2h.elapsed();
+    return RTM_HIP_OK;
+}
+
+int backward_impl(rtm_hip_engine* h, const rtm_hip_shot* s) {
+    if (int rc = validate_shot(h, s, true)) return rc;
+    rtm::Timer wall;
+    const int tile_x = h->tile_x();
+    if (int rc = build_entries(h, 1, s, false, true, tile_x)) return rc;
+    if (s->nrec > 0) {
+        const size_t n = (size_t)s->nrec * s->nt;
+        if (int rc = h->grow(&h->d_traces, &h->traces_cap, n, "trace buffer")) return rc;
+        rtm::Timer t;
+        hipError_t e = hipMemcpyAsync(h->d_traces, s->traces, n * sizeof(float), hipMemcpyHostToDevice,
+                                      h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "upload of traces");
+        e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "upload of traces");
+        h->times.h2d += t.elapsed();
+    }
+    if (int rc = run_loop(h, Dir::Backward, s, false, tile_x)) return rc;
+    h->times.backward += wall.elapsed();
+    return RTM_HIP_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int rtm_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rtm_hip_create(int device, rtm_hip_engine** out) {
+    if (!out) {
+        g_null_handle_error = "null output pointer";
+        return RTM_HIP_ERR_ARG;
+    }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_null_handle_error = "no HIP device available";
+        return RTM_HIP_ERR_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_null_handle_error = "device index out of range";
+        return RTM_HIP_ERR_ARG;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        g_null_handle_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return RTM_HIP_ERR_DEVICE;
+    }
+    auto* h = new (std::nothrow) rtm_hip_engine();
+    if (!h) {
+        g_null_handle_error = "out of host memory";
+        return RTM_HIP_ERR_MEMORY;
+    }
+    h->device = device;
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_null_handle_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete h;
+        return RTM_HIP_ERR_DEVICE;
+    }
+    *out = h;
+    return RTM_HIP_OK;
+}
+
+void rtm_hip_destroy(rtm_hip_engine* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char* rtm_hip_last_error(const rtm_hip_engine* h) {
+    return h ? h->err.c_str() : g_null_handle_error.c_str();
+}
+
+int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, const rtm_hip_params* p,
+                  int nt, float dt) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (!g || !v || !p) return h->fail(RTM_HIP_ERR_ARG, "null grid, velocity or params");
+    // validation of CPUReferenceRTM::setup (src/rtm/rtm_cpu.cpp:15-22)
+    const int order = p->order;
+    if (order != 2 && order != 4 && order != 6 && order != 8 && order != 10 && order != 12)
+        return h->fail(RTM_HIP_ERR_ARG, "unsupported spatial order (use 2, 4, 6, 8, 10 or 12)");
+    const int half = order / 2;
+    if (p->nb < half + 1) return h->fail(RTM_HIP_ERR_ARG, "--nb must be at least order/2 + 1");
+    if (g->nx <= 0 || g->nz <= 0) return h->fail(RTM_HIP_ERR_ARG, "velocity model size does not match nx*nz");
+    if (p->store_interval < 1) return h->fail(RTM_HIP_ERR_ARG, "--store-interval must be >= 1");
+    if (nt < 1) return h->fail(RTM_HIP_ERR_ARG, "nt must be >= 1");
+    if (!(dt > 0.0f)) return h->fail(RTM_HIP_ERR_ARG, "dt must be > 0");
+    if (!(g->dx > 0.0f) || !(g->dz > 0.0f)) return h->fail(RTM_HIP_ERR_ARG, "dx and dz must be > 0");
+    if (p->snapshot_format != RTM_HIP_SNAP_F32)
+        return h->fail(RTM_HIP_ERR_ARG, "snapshot format %d is not available in this build",
+                       p->snapshot_format);
+    const long long nxe = (long long)g->nx + 2LL * p->nb, nze = (long long)g->nz + 2LL * p->nb;
+    const long long ldz = round_up((int)nze, rtm_hip::kTileZ);
+    if (nxe * ldz >= (1LL << 31)) return h->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
+
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    (void)hipStreamSynchronize(h->stream);
+    h->release();
+
+    h->nx = g->nx; h->nz = g->nz; h->dx = g->dx; h->dz = g->dz; h->ox = g->ox; h->oz = g->oz;
+    h->nb = p->nb; h->order = order; h->half = half;
+    h->nxe = (int)nxe; h->nze = (int)nze; h->ldz = (int)ldz;
+    h->nt = nt; h->dt = dt; h->store = p->store_interval; h->verbose = p->verbose;
+    h->snap_fmt = p->snapshot_format;
+    h->nsnap = (nt - 1) / p->store_interval + 1;
+    h->n_int = (size_t)g->nx * g->nz;
+    h->n_fld = (size_t)nxe * ldz;
+    h->snap_bytes = (size_t)h->nsnap * h->n_int * sizeof(float);
+    h->times = rtm_hip_times{};
+
+    rtm_hip::HostTables t;
+    try {
+        t = rtm_hip::build_tables(g->nx, g->nz, g->dx, g->dz, p->nb, order, p->sponge_alpha, p->f0, nt, dt,
+                                  v, h->ldz);
+    } catch (const std::exception& ex) {
+        return h->fail(RTM_HIP_ERR_ARG, "%s", ex.what());
+    }
+    h->coef = t.coef;
+
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const size_t need = h->snap_bytes + 5 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float);
+        if (need > free_b)
+            return h->fail(RTM_HIP_ERR_MEMORY,
+                           "engine needs %.2f GiB of device memory (snapshots %.2f GiB), %.2f GiB free; "
+                           "raise --store-interval",
+                           need / 1073741824.0, h->snap_bytes / 1073741824.0, free_b / 1073741824.0);
+    }
+
+    int rc = RTM_HIP_OK;
+    if (!rc) rc = h->alloc(&h->d_vdt2, h->n_fld, "vdt2");
+    if (!rc) rc = h->alloc(&h->d_wx, (size_t)h->nxe, "wx");
+    if (!rc) rc = h->alloc(&h->d_wz, (size_t)h->ldz, "wz");
+    if (!rc) rc = h->alloc(&h->d_wavelet, (size_t)nt, "wavelet");
+    for (int b = 0; b < 3 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");
+    if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
+    if (!rc) rc = h->alloc(&h->d_image, h->n_int, "image");
+    if (!rc) rc = h->alloc(&h->d_illum, h->n_int, "illumination");
+    if (rc) {
+        h->release();
+        return rc;
+    }
+    rtm::Timer h2d;
+    e = hipMemcpy(h->d_vdt2, t.vdt2.data(), t.vdt2.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_wx, t.wx.data(), t.wx.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_wz, t.wz.data(), t.wz.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->d_wavelet, t.wavelet.data(), t.wavelet.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(h->d_image, 0, h->n_int * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(h->d_illum, 0, h->n_int * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(h->d_snap, 0, h->snap_bytes);
+    if (e != hipSuccess) {
+        h->release();
+        return h->hip_fail(e, "table upload");
+    }
+    h->times.h2d += h2d.elapsed();
+    h->ready = true;
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (!out) return const_cast<rtm_hip_engine*>(h)->fail(RTM_HIP_ERR_ARG, "null info");
+    std::memset(out, 0, sizeof *out);
+    out->nx = h->nx; out->nz = h->nz; out->nb = h->nb; out->half = h->half;
+    out->nxe = h->nxe; out->nze = h->nze; out->ldz = h->ldz;
+    out->nt = h->nt; out->store_interval = h->store; out->num_snapshots = h->nsnap;
+    out->snapshot_bytes = h->snap_bytes;
+    out->device_bytes = h->device_bytes;
+    out->snapshot_format = h->snap_fmt;
+    out->tile_x = h->ready ? h->tile_x() : 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
+        std::snprintf(out->device_name, sizeof out->device_name, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_forward(rtm_hip_engine* h, const rtm_hip_shot* shot, float* snapshots_out, float* recorded) {
+    if (int rc = check_handle(h, true)) return rc;
+    (void)hipSetDevice(h->device);
+    return forward_impl(h, shot, snapshots_out, recorded);
+}
+
+int rtm_hip_backward(rtm_hip_engine* h, const rtm_hip_shot* shot) {
+    if (int rc = check_handle(h, true)) return rc;
+    (void)hipSetDevice(h->device);
+    return backward_impl(h, shot);
+}
+
+int rtm_hip_image_zero(rtm_hip_engine* h) {
+    if (int rc = check_handle(h, true)) return rc;
+    (void)hipSetDevice(h->device);
+    hipError_t e = hipMemsetAsync(h->d_image, 0, h->n_int * sizeof(float), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->d_illum, 0, h->n_int * sizeof(float), h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image reset");
+}
+
+int rtm_hip_image_get(rtm_hip_engine* h, float* image, float* illum) {
+    if (int rc = check_handle(h, true)) return rc;
+    (void)hipSetDevice(h->device);
+    rtm::Timer t;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && image)
+        e = hipMemcpy(image, h->d_image, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && illum)
+        e = hipMemcpy(illum, h->d_illum, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+    h->times.d2h += t.elapsed();
+    return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image download");
+}
+
+int rtm_hip_migrate(rtm_hip_engine* h, int nshots, const rtm_hip_shot* shots, float* image, float* illum) {
+    if (int rc = check_handle(h, true)) return rc;
+    if (nshots < 0 || (nshots > 0 && !shots)) return h->fail(RTM_HIP_ERR_ARG, "bad shot list");
+    rtm::Timer total;
+    if (int rc = rtm_hip_image_zero(h)) return rc;
+    for (int is = 0; is < nshots; ++is) {
+        if (h->verbose) {
+            std::printf("  shot %2d / %2d   sx=%8.1f m  sz=%6.1f m  nrec=%d\n", is + 1, nshots, shots[is].sx,
+                        shots[is].sz, shots[is].nrec);
+            std::fflush(stdout);
+        }
+        if (int rc = forward_impl(h, &shots[is], nullptr, nullptr)) return rc;
+        if (int rc = backward_impl(h, &shots[is])) return rc;
+    }
+    if (int rc = rtm_hip_image_get(h, image, illum)) return rc;
+    h->times.total += total.elapsed();
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_model_shot(rtm_hip_engine* h, const rtm_hip_shot* shot, float* traces_out) {
+    if (int rc = check_handle(h, true)) return rc;
+    if (!traces_out) return h->fail(RTM_HIP_ERR_ARG, "null trace output");
+    (void)hipSetDevice(h->device);
+    if (shot && shot->nrec == 0) return RTM_HIP_OK;
+    return forward_impl(h, shot, nullptr, traces_out);
+}
+
+int rtm_hip_get_times(const rtm_hip_engine* h, rtm_hip_times* out) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (out) *out = h->times;
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_reset_times(rtm_hip_engine* h) {
+    if (int rc = check_handle(h, false)) return rc;
+    h->times = rtm_hip_times{};
+    h->launches[0] = h->launches[1] = 0;
+    h->sampled[0] = h->sampled[1] = 0;
+    h->sampled_ms[0] = h->sampled_ms[1] = 0.0;
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (every < 0) return h->fail(RTM_HIP_ERR_ARG, "sampling interval must be >= 0");
+    h->timing_every = every;
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (!out) return RTM_HIP_OK;
+    for (int k = 0; k < 2; ++k) {
+        out->launches[k] = h->launches[k];
+        out->sampled[k] = h->sampled[k];
+        out->sampled_ms[k] = h->sampled_ms[k];
+    }
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_set_tile(rtm_hip_engine* h, int tile_x) {
+    if (int rc = check_handle(h, false)) return rc;
+    if (tile_x != 0 && tile_x != 4 && tile_x != 8 && tile_x != 16)
+        return h->fail(RTM_HIP_ERR_ARG, "tile width must be 0, 4, 8 or 16");
+    h->tile_x_user = tile_x;
+    return RTM_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// step_kernels.hip -- the fused leapfrog step for gfx950 (CDNA4, wave64).
+//
+// One launch advances one shot by one time step and does, in a single pass
+// over the extended grid, what the reference does in up to five passes:
+//
+//   fd_time_step      src/rtm/forward.cpp:29-50    order-2H Laplacian + leapfrog
+//   apply_sponge      src/rtm/forward.cpp:56-63    Cerjan damping of p^n and p^{n+1}
+//   inject_source     src/rtm/forward.cpp:70-72    forward
+//   inject_receivers  src/rtm/backward.cpp:20-27   backward
+//   record_traces     src/rtm/forward.cpp:75-79    forward modelling
+//   save_snapshot     src/rtm/forward.cpp:83-88    forward, stored steps
+//   imaging           src/rtm/imaging.cpp:25-40    backward, stored steps
+//
+// Exactness.  Every float operation of the reference is performed once, in
+// the reference's order, with no contraction (built -ffp-contract=off) and
+// with FP32 denormals preserved, so the result is bit-identical:
+//   lap = c0*p; lap = lap + (cx_k*(x+ + x-) + cz_k*(z+ + z-)),  k = 1..H
+//   p^{n+1} = ((2 p^n - p^{n-1}*w) + vdt2*lap) * w,   w = wx[ix]*wz[iz]
+// The reference damps p^n in place after the stencil (forward.cpp:56-63); that
+// multiply is applied here when the buffer is read back as p^{n-1} on the next
+// step (p_prev*w), so no thread ever writes a cell another workgroup reads in
+// the same launch.  Injection and recording cells are interior cells, where
+// w == 1.0f exactly, so injecting after the damped write is the reference's
+// inject-then-damp.  Duplicate receiver cells add in ascending receiver order
+// through a per-tile entry table, which is the reference's sequential order.
+//
+// Mapping.  A 256-thread workgroup (4 x wave64) owns a tile of 64 depth cells
+// x tile_x columns.  The tile of p^n plus an H-cell halo is staged in LDS with
+// 256-byte coalesced row loads (the device pitch ldz is a multiple of 64
+// floats).  Each lane keeps its column's tile_x/4 + 2H values of p^n in
+// registers, so x-neighbours cost no LDS traffic; z-neighbours are LDS reads
+// of adjacent lanes (bank-conflict free).  p^{n-1}, vdt2 and p^{n+1} are
+// streamed once.  Workgroups are numbered so that each of the 8 XCDs gets a
+// contiguous band of tiles, keeping halo re-reads inside one XCD's L2.
+#include "step_kernels.hpp"
+
+namespace rtm_hip {
+
+namespace {
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Give XCD k the
+// k-th contiguous block of logical tiles so neighbouring tiles share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+    const int q = ntiles >> 3, r = ntiles & 7;
+    const int xcd = b & 7, loc = b >> 3;
+    return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+template <int H, int RX, Dir D>
+__global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
+    constexpr int TX = RX * kWaves;          // tile columns
+    constexpr int LZ = kTileZ + 2 * H;       // LDS row length (depth + halo)
+    constexpr int LX = TX + 2 * H;           // LDS rows (columns + halo)
+    __shared__ float tile[LX * LZ];
+
+    const int t = xcd_tile((int)blockIdx.x, a.ntiles);
+    const int tz = t % a.ntz, tx = t / a.ntz;
+    const int z0 = tz * kTileZ, x0 = tx * TX;
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const int wave = (int)threadIdx.x / kTileZ;
+
+    // ---- stage p^n (tile + halo) in LDS; out-of-grid cells read as 0 -------
+    for (int r = wave; r < LX; r += kWaves) {
+        const int gx = x0 - H + r;
+        const bool xin = gx >= 0 && gx < a.nxe;
+        const float* row = a.p_cur + (ptrdiff_t)(xin ? gx : 0) * a.ldz;
+        tile[r * LZ + H + lane] = xin ? row[z0 + lane] : 0.0f;
+        if (lane < 2 * H) {
+            const int hz = lane < H ? z0 - H + lane : z0 + kTileZ + (lane - H);
+            const int col = lane < H ? lane : kTileZ + lane;
+            const bool zin = hz >= 0 && hz < a.ldz;
+            tile[r * LZ + col] = (xin && zin) ? row[zin ? hz : 0] : 0.0f;
+        }
+    }
+    __syncthreads();
+
+    const int iz = z0 + lane;
+    const bool z_ok = iz >= H && iz < a.nze - H;
+    const int lx0 = wave * RX;               // first local column of this wave
+
+    // this lane's depth row of p^n across its columns and the x-halo
+    float q[RX + 2 * H];
+#pragma unroll
+    for (int j = 0; j < RX + 2 * H; ++j) q[j] = tile[(lx0 + j) * LZ + H + lane];
+
+    const float wzv = a.wz[iz];
+    const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+        const int ix = x0 + lx0 + r;
+        if (!z_ok || ix < H || ix >= a.nxe - H) continue;
+
+        const float* T = tile + (lx0 + r + H) * LZ + H + lane;
+        const float pc = q[r + H];
+        float lap = a.coef.c0 * pc;
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            const float tx_k = a.coef.cx[k] * (q[r + H + k] + q[r + H - k]);
+            const float tz_k = a.coef.cz[k] * (T[k] + T[-k]);
+            lap = lap + (tx_k + tz_k);
+        }
+
+        const ptrdiff_t i = (ptrdiff_t)ix * a.ldz + iz;
+        const float w = a.wx[ix] * wzv;
+        const float v2 = a.vdt2[i];
+        const float prev = a.p_prev[i] * w;
+        float out = (2.0f * pc - prev) + v2 * lap;
+        out = out * w;
+
+        // injection / recording entries of this tile (usually none)
+        for (int e = e0; e < e1; ++e) {
+            const Entry en = a.entries[e];
+            if (en.cell != (int)i) continue;
+            if constexpr (D == Dir::Forward) {
+                if (en.code < 0) out = out + v2 * a.wavelet[a.it];
+                else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
+            } else {
+                out = out + v2 * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
+            }
+        }
+        a.p_next[i] = out;
+
+        const int jx = ix - a.nb, jz = iz - a.nb;
+        if (jx >= 0 && jx < a.nx && jz >= 0 && jz < a.nz) {
+            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+            if constexpr (D == Dir::Forward) {
+                if (a.snap_out) a.snap_out[j] = out;
+            } else {
+                if (a.snap_in) {
+                    const float f = a.snap_in[j];
+                    a.image[j] = a.image[j] + f * out;
+                    a.illum[j] = a.illum[j] + f * f;
+                }
+            }
+        }
+    }
+}
+
+template <int H, Dir D>
+hipError_t launch_h(int tile_x, const StepArgs& a, hipStream_t s) {
+    const dim3 block(kBlockThreads);
+    const dim3 grid((unsigned)a.ntiles);
+    switch (tile_x) {
+        case 4:  hipLaunchKernelGGL((k_step<H, 1, D>), grid, block, 0, s, a); break;
+        case 8:  hipLaunchKernelGGL((k_step<H, 2, D>), grid, block, 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_step<H, 4, D>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <Dir D>
+hipError_t launch_dir(int half, int tile_x, const StepArgs& a, hipStream_t s) {
+    switch (half) {
+        case 1: return launch_h<1, D>(tile_x, a, s);
+        case 2: return launch_h<2, D>(tile_x, a, s);
+        case 3: return launch_h<3, D>(tile_x, a, s);
+        case 4: return launch_h<4, D>(tile_x, a, s);
+        case 5: return launch_h<5, D>(tile_x, a, s);
+        case 6: return launch_h<6, D>(tile_x, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_step(Dir dir, int half, int tile_x, const StepArgs& a, hipStream_t stream) {
+    return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, tile_x, a, stream)
+                               : launch_dir<Dir::Backward>(half, tile_x, a, stream);
+}
+
+}  // namespace rtm_hip

@@ -1,0 +1,52 @@
+"""HIP engine vs the CPU oracle on small grids: bit-identical traces and images.
+
+The oracle (oracle/liboracle.so) is itself bit-identical to the reference's
+-O2 build (tests/test_oracle_vs_reference.py), so equality here is equality
+with the reference's CPU path.  Each case prints its parameters first so a
+failure names it.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal, line_shots, small_model
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(order, store, nt=160, dt=0.001, tile=0):
+    model = small_model()
+    import rtm_amd
+    params = rtm_amd.RTMParams(nb=max(order // 2 + 1, 12), order=order, store_interval=store, f0=15.0)
+    shots = line_shots(model, nt, dt, [100.0, 250.0])
+    return model, params, shots
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8, 10, 12])
+def test_model_shot_bit_identical(hip_engine_cls, order):
+    print(f"case: model_shot order={order}", flush=True)
+    model, params, shots = _case(order, 1)
+    orc = O.Oracle(model, params, shots[0].nt, shots[0].dt)
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, shots[0].nt, shots[0].dt)
+        for s in shots:
+            ref = orc.model_shot(s)
+            got = eng.model_shot(s)
+            assert bits_equal(ref, got), f"max diff {np.abs(ref - got).max()}"
+
+
+@pytest.mark.parametrize("order,store,tile", [(2, 1, 4), (4, 3, 8), (6, 2, 16), (8, 2, 0),
+                                              (10, 5, 4), (12, 1, 8), (8, 7, 16)])
+def test_migrate_bit_identical(hip_engine_cls, order, store, tile):
+    print(f"case: migrate order={order} store={store} tile={tile}", flush=True)
+    model, params, shots = _case(order, store)
+    orc = O.Oracle(model, params, shots[0].nt, shots[0].dt)
+    for s in shots:
+        s.traces = orc.model_shot(s)
+    I_ref, S_ref = orc.migrate(shots)
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, shots[0].nt, shots[0].dt)
+        eng.set_tile(tile)
+        I, S = eng.migrate(shots)
+    assert bits_equal(I_ref, I), f"image max diff {np.abs(I_ref - I).max()} of {np.abs(I_ref).max()}"
+    assert bits_equal(S_ref, S)
