@@ -1,0 +1,45 @@
+"""Error behaviour of the C ABI (the reference throws std::runtime_error for
+the same conditions, src/rtm/rtm_cpu.cpp:15-22, src/rtm/rtm.cpp:33-34)."""
+import numpy as np
+import pytest
+
+from conftest import small_model
+
+pytestmark = pytest.mark.gpu
+
+
+def test_setup_validation(hip_engine_cls):
+    import rtm_amd
+    m = small_model()
+    with hip_engine_cls() as eng:
+        with pytest.raises(RuntimeError, match="unsupported spatial order"):
+            eng.setup(m, rtm_amd.RTMParams(nb=20, order=7), 10, 0.001)
+        with pytest.raises(RuntimeError, match="nb must be at least"):
+            eng.setup(m, rtm_amd.RTMParams(nb=4, order=8), 10, 0.001)
+        with pytest.raises(RuntimeError, match="store-interval"):
+            eng.setup(m, rtm_amd.RTMParams(nb=20, order=8, store_interval=0), 10, 0.001)
+        with pytest.raises(RuntimeError, match="not set up"):
+            eng.image()
+        with pytest.raises(RuntimeError, match="snapshot format"):
+            eng.setup(m, rtm_amd.RTMParams(nb=20, order=8), 10, 0.001, snapshot_format=9)
+        with pytest.raises(RuntimeError, match="tile width"):
+            eng.set_tile(5)
+
+
+def test_shot_validation(hip_engine_cls):
+    import rtm_amd
+    m = small_model()
+    rx = np.arange(0, 300, 10, dtype=np.float32)
+    with hip_engine_cls() as eng:
+        eng.setup(m, rtm_amd.RTMParams(nb=12, order=8), 100, 0.001)
+        short = rtm_amd.ShotRecord(100.0, 20.0, rx, np.full_like(rx, 20.0), 50, 0.001,
+                                   np.zeros((rx.size, 50), np.float32))
+        with pytest.raises(RuntimeError, match="fewer than the engine's nt"):
+            eng.backward(short)
+        no_data = rtm_amd.ShotRecord(100.0, 20.0, rx, np.full_like(rx, 20.0), 100, 0.001)
+        with pytest.raises(RuntimeError, match="needs traces"):
+            eng.backward(no_data)
+        # the engine keeps working after refused calls
+        ok = rtm_amd.ShotRecord(100.0, 20.0, rx, np.full_like(rx, 20.0), 100, 0.001)
+        tr = eng.model_shot(ok)
+        assert np.abs(tr).max() > 0

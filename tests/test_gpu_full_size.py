@@ -1,0 +1,65 @@
+"""The HIP engine at the 2301 x 751 workload's full size.
+
+One shot of the real shape (nt = 8000, order 8, nb 60, store 4, 767
+receivers) is modelled and migrated on the GPU and by the CPU oracle
+(cpu-opt, all host cores): traces and image must be bit-identical.  Then
+size-independent properties: a second run is bit-identical, doubling the data
+doubles the image (to FP32 subnormal rounding), and the illumination does not
+depend on the data."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def workload():
+    import rtm_amd
+    from rtm_amd import models
+    model = models.faulted_layers(2301, 751, 4.0, 4.0)
+    params = rtm_amd.RTMParams(nb=60, order=8, store_interval=4, f0=15.0)
+    nt, dt = 8000, 0.00035
+    shot = models.surface_shots(model, nt, dt, [48.0 * 96], rec_dx=12.0)[0]
+    assert shot.nrec == 767
+    return model, params, nt, dt, shot
+
+
+def test_full_size_bit_identical_to_oracle(hip_engine_cls, workload):
+    model, params, nt, dt, shot = workload
+    print("case: 2301x751 nt=8000 one shot vs oracle cpu-opt", flush=True)
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, nt, dt)
+        traces = eng.model_shot(shot)
+        shot.traces = traces
+        image, illum = eng.migrate([shot])
+    orc = O.Oracle(model, params, nt, dt, variant="cpu-opt")
+    assert bits_equal(orc.model_shot(shot), traces)
+    I_ref, S_ref = orc.migrate([shot])
+    assert bits_equal(image, I_ref), f"rel L2 {np.linalg.norm(image - I_ref) / np.linalg.norm(I_ref):.3e}"
+    assert bits_equal(illum, S_ref)
+
+
+def test_full_size_properties(hip_engine_cls, workload):
+    import rtm_amd
+    model, params, nt, dt, shot = workload
+    print("case: 2301x751 determinism + linearity", flush=True)
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, nt, dt)
+        if shot.traces is None:
+            shot.traces = eng.model_shot(shot)
+        I1, S1 = eng.migrate([shot])
+        I1b, S1b = eng.migrate([shot])
+        doubled = rtm_amd.ShotRecord(shot.sx, shot.sz, shot.rx, shot.rz, shot.nt, shot.dt,
+                                     shot.traces * np.float32(2))
+        I2, S2 = eng.migrate([doubled])
+    assert bits_equal(I1, I1b) and bits_equal(S1, S1b)
+    assert bits_equal(S1, S2)
+    # Doubling is exact except where the receiver field passes through FP32
+    # subnormals (deep in the sponge), whose absolute rounding does not scale.
+    d = I2.astype(np.float64) - 2.0 * I1.astype(np.float64)
+    assert np.abs(I1).max() > 0
+    assert np.linalg.norm(d) <= 1e-7 * np.linalg.norm(2.0 * I1)
+    assert np.abs(d).max() <= 1e-6 * 2.0 * np.abs(I1).max()
