@@ -92,7 +92,10 @@ class HipEngine:
 
     @property
     def n_int(self) -> int:
-        return self.model.nx * self.model.nz
+        """Interior cells of the set-up grid (0 before setup)."""
+        i = _capi.Info()
+        self._check(self.lib.rtm_hip_get_info(self.h, C.byref(i)))
+        return int(i.nx) * int(i.nz)
 
     # ---- propagation ------------------------------------------------------
     def forward(self, shot: ShotRecord, snapshots: bool = False, record: bool = False):
@@ -113,8 +116,11 @@ class HipEngine:
         self._check(self.lib.rtm_hip_image_zero(self.h))
 
     def image(self):
-        img = np.empty(self.n_int, np.float32)
-        ill = np.empty(self.n_int, np.float32)
+        n = self.n_int
+        if n == 0:    # not set up: let the engine report it
+            self._check(self.lib.rtm_hip_image_get(self.h, None, None))
+        img = np.empty(n, np.float32)
+        ill = np.empty(n, np.float32)
         self._check(self.lib.rtm_hip_image_get(self.h, _capi.fptr(img), _capi.fptr(ill)))
         return img, ill
 
@@ -123,8 +129,9 @@ class HipEngine:
         arr = (_capi.Shot * max(len(shots), 1))()
         for i, sh in enumerate(shots):
             arr[i] = _capi.shot_struct(sh, keep)
-        img = np.empty(self.n_int, np.float32)
-        ill = np.empty(self.n_int, np.float32)
+        n = self.n_int
+        img = np.empty(max(n, 1), np.float32)
+        ill = np.empty(max(n, 1), np.float32)
         self._check(self.lib.rtm_hip_migrate(self.h, len(shots), arr, _capi.fptr(img), _capi.fptr(ill)))
         return img, ill
 
