@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--family", type=int, default=-1, help="step-kernel family (-1 built-in, 0 LDS tile, 1 march)")
     ap.add_argument("--tile", type=int, default=0, help="step-kernel tile width (0 = built-in)")
     ap.add_argument("--sample-every", type=int, default=32, help="event-time every N-th step launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -121,8 +122,7 @@ def main():
 
     eng = rtm_amd.HipEngine(0)
     eng.setup(model, params, nt, dt)
-    if args.tile:
-        eng.set_tile(args.tile)
+    eng.set_step_kernel(args.family, args.tile)
     print(f"bench: device {eng.info()['device_name']}, modelling {K + W} shots", file=sys.stderr, flush=True)
     for s in shots:
         s.traces = eng.model_shot(s)
@@ -160,7 +160,8 @@ def main():
         "data": "synthetic (seeded faulted-layer model; traces modelled on the GPU)",
         "config": {"workload": "2301x751 @4 m, nt 8000, dt 0.35 ms, order 8, nb 60, store 4, "
                                "767 receivers, plain per-shot engine, FP32 snapshots",
-                   "engine": "hip", "tile_x": info["tile_x"], "ldz": info["ldz"]},
+                   "engine": "hip", "step_family": ["lds-tile", "march"][info["step_family"]],
+                   "tile_x": info["tile_x"], "ldz": info["ldz"]},
         "shots_per_s": round(K / elapsed, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},

@@ -35,6 +35,10 @@ extern "C" {
 #define RTM_HIP_ERR_DEVICE   -2   /* HIP runtime error (incl. no device) */
 #define RTM_HIP_ERR_MEMORY   -3   /* device or host allocation failed    */
 
+/* Step-kernel families (rtm_hip_set_step_kernel). */
+#define RTM_HIP_STEP_LDS_TILE  0
+#define RTM_HIP_STEP_MARCH     1
+
 /* Snapshot storage formats (rtm_hip_setup's params->snapshot_format). */
 #define RTM_HIP_SNAP_F32   0      /* the reference's format (src/rtm/forward.cpp:83-88) */
 #define RTM_HIP_SNAP_F16   1      /* IEEE half + one FP32 scale per snapshot            */
@@ -82,6 +86,7 @@ typedef struct {
     size_t snapshot_bytes;       /* device bytes of the snapshot store   */
     size_t device_bytes;         /* all device buffers of this handle    */
     int    snapshot_format;
+    int    step_family;          /* RTM_HIP_STEP_* of the step kernel    */
     int    tile_x;               /* step-kernel tile width along x       */
     char   device_name[128];
 } rtm_hip_info;
@@ -166,11 +171,13 @@ int         rtm_hip_reset_times(rtm_hip_engine* h);
 int         rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every);
 int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out);
 
-/* Step-kernel tile width along x (cells per 256-thread workgroup: 4, 8 or 16);
- * 0 restores the built-in choice.  Takes effect at the next forward/backward.
- * No reference counterpart (the CUDA block shape is fixed,
- * src/cuda/propagation_cuda.cu:96-97). */
-int         rtm_hip_set_tile(rtm_hip_engine* h, int tile_x);
+/* Step-kernel mapping: family RTM_HIP_STEP_MARCH (one wave64 per depth strip,
+ * all loads up front) or RTM_HIP_STEP_LDS_TILE (LDS-staged 64-cell tiles), or
+ * -1 for the built-in choice; tile_x = output columns per tile (4, 8 or 16),
+ * 0 for the built-in choice.  Results are identical for every choice; takes
+ * effect at the next forward/backward.  No reference counterpart (the CUDA
+ * block shape is fixed, src/cuda/propagation_cuda.cu:96-97). */
+int         rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x);
 
 /* Replaces rtm::mute_direct_wave (include/rtm_io.hpp:9; src/io/seismic_io.cpp:18-35),
  * host-only: traces (nrec*nt, stride shot->nt) muted in place. */

@@ -54,7 +54,8 @@ struct rtm_hip_engine {
     float dx = 0, dz = 0, ox = 0, oz = 0, dt = 0;
     size_t n_int = 0, n_fld = 0;
     rtm_hip::Coeffs coef{};
-    int tile_x_user = 0;
+    int family_user = -1;     // -1: built-in choice
+    int tile_x_user = 0;      // 0: built-in choice
 
     // device buffers
     float* d_vdt2 = nullptr;
@@ -165,12 +166,17 @@ struct rtm_hip_engine {
         ready = false;
     }
 
-    int tile_x() const {
-        if (tile_x_user) return tile_x_user;
-        // Enough workgroups to fill 256 CUs several times over, wider tiles
-        // (less halo) once the grid is large.
-        const long long cells = (long long)nxe * ldz;
-        return cells >= 4LL * 1024 * 1024 ? 16 : (cells >= 512LL * 1024 ? 8 : 4);
+    // Step-kernel tiling: the march mapping with 16-column tiles unless the
+    // caller chose otherwise; small grids take narrower tiles so the launch
+    // still spreads over every CU.
+    rtm_hip::StepGeometry geometry() const {
+        const auto fam = family_user < 0 ? rtm_hip::Family::kMarch : static_cast<rtm_hip::Family>(family_user);
+        int tx = tile_x_user;
+        if (tx == 0) {
+            const long long cells = (long long)nxe * ldz;
+            tx = cells >= 1024LL * 1024 ? 16 : (cells >= 256LL * 1024 ? 8 : 4);
+        }
+        return rtm_hip::step_geometry(fam, half, tx, nxe, nze, ldz);
     }
 };
 
@@ -194,10 +200,8 @@ CellIndex to_cell(const rtm_hip_engine* h, float x, float z, int* nclamped) {
 // cell, then code, so at a cell the source (code -1) precedes recordings and
 // duplicate receivers keep ascending receiver order (src/rtm/backward.cpp:23-26).
 int build_entries(rtm_hip_engine* h, int k, const rtm_hip_shot* s, bool with_source,
-                  bool with_receivers, int tile_x) {
-    const int ntz = rtm_hip::step_tiles_z(h->ldz);
-    const int ntx = rtm_hip::step_tiles_x(h->nxe, tile_x);
-    const int ntiles = ntz * ntx;
+                  bool with_receivers, const rtm_hip::StepGeometry& g) {
+    const int ntiles = g.ntiles;
     int nclamped = 0;
     struct Tagged {
         int tile, cell, code;
@@ -205,7 +209,7 @@ int build_entries(rtm_hip_engine* h, int k, const rtm_hip_shot* s, bool with_sou
     std::vector<Tagged> v;
     v.reserve(static_cast<size_t>(s->nrec) + 1);
     auto add = [&](CellIndex c, int code) {
-        const int tile = (c.ixe / tile_x) * ntz + c.ize / rtm_hip::kTileZ;
+        const int tile = rtm_hip::tile_of_cell(g, c.ixe, c.ize);
         v.push_back({tile, c.ixe * h->ldz + c.ize, code});
     };
     const CellIndex src = to_cell(h, s->sx, s->sz, &nclamped);
@@ -301,7 +305,8 @@ int validate_shot(rtm_hip_engine* h, const rtm_hip_shot* s, bool need_traces) {
 }
 
 // Runs one propagation (forward or backward) of h->nt steps.
-int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record, int tile_x) {
+int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
+             const rtm_hip::StepGeometry& g) {
     const int k = dir == Dir::Forward ? 0 : 1;
     rtm_hip::StepArgs a{};
     a.vdt2 = h->d_vdt2;
@@ -309,9 +314,9 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record, int
     a.wz = h->d_wz;
     a.coef = h->coef;
     a.nxe = h->nxe; a.nze = h->nze; a.ldz = h->ldz; a.nb = h->nb; a.nx = h->nx; a.nz = h->nz;
-    a.ntz = rtm_hip::step_tiles_z(h->ldz);
-    a.ntx = rtm_hip::step_tiles_x(h->nxe, tile_x);
-    a.ntiles = a.ntz * a.ntx;
+    a.ntz = g.ntz;
+    a.ntx = g.ntx;
+    a.ntiles = g.ntiles;
     a.tile_ptr = h->d_tile_ptr[k];
     a.entries = h->d_ent[k];
     a.wavelet = h->d_wavelet;
@@ -349,7 +354,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record, int
             ev = take_event(h);
             (void)hipEventRecord(ev.start, h->stream);
         }
-        hipError_t e = rtm_hip::launch_step(dir, h->half, tile_x, a, h->stream);
+        hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
         if (sample) {
             (void)hipEventRecord(ev.stop, h->stream);
@@ -375,16 +380,16 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record, int
 int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out, float* recorded) {
     if (int rc = validate_shot(h, s, false)) return rc;
     rtm::Timer wall;
-    const int tile_x = h->tile_x();
+    const rtm_hip::StepGeometry g = h->geometry();
     const bool record = recorded != nullptr && s->nrec > 0;
-    if (int rc = build_entries(h, 0, s, true, record, tile_x)) return rc;
+    if (int rc = build_entries(h, 0, s, true, record, g)) return rc;
     if (record) {
         const size_t n = (size_t)s->nrec * h->nt;
         if (int rc = h->grow(&h->d_rec, &h->rec_cap, n, "recorded traces")) return rc;
         hipError_t e = hipMemsetAsync(h->d_rec, 0, n * sizeof(float), h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "recorded traces reset");
     }
-    if (int rc = run_loop(h, Dir::Forward, s, record, tile_x)) return rc;
+    if (int rc = run_loop(h, Dir::Forward, s, record, g)) return rc;
     h->times.forward += wall.elapsed();
 
     rtm::Timer d2h;
@@ -405,8 +410,8 @@ int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out,
 int backward_impl(rtm_hip_engine* h, const rtm_hip_shot* s) {
     if (int rc = validate_shot(h, s, true)) return rc;
     rtm::Timer wall;
-    const int tile_x = h->tile_x();
-    if (int rc = build_entries(h, 1, s, false, true, tile_x)) return rc;
+    const rtm_hip::StepGeometry g = h->geometry();
+    if (int rc = build_entries(h, 1, s, false, true, g)) return rc;
     if (s->nrec > 0) {
         const size_t n = (size_t)s->nrec * s->nt;
         if (int rc = h->grow(&h->d_traces, &h->traces_cap, n, "trace buffer")) return rc;
@@ -418,7 +423,7 @@ int backward_impl(rtm_hip_engine* h, const rtm_hip_shot* s) {
         if (e != hipSuccess) return h->hip_fail(e, "upload of traces");
         h->times.h2d += t.elapsed();
     }
-    if (int rc = run_loop(h, Dir::Backward, s, false, tile_x)) return rc;
+    if (int rc = run_loop(h, Dir::Backward, s, false, g)) return rc;
     h->times.backward += wall.elapsed();
     return RTM_HIP_OK;
 }
@@ -584,7 +589,11 @@ int rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out) {
     out->snapshot_bytes = h->snap_bytes;
     out->device_bytes = h->device_bytes;
     out->snapshot_format = h->snap_fmt;
-    out->tile_x = h->ready ? h->tile_x() : 0;
+    if (h->ready) {
+        const rtm_hip::StepGeometry g = h->geometry();
+        out->tile_x = g.tile_x;
+        out->step_family = static_cast<int>(g.family);
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
         std::snprintf(out->device_name, sizeof out->device_name, "%s (%s)", prop.name, prop.gcnArchName);
@@ -686,10 +695,14 @@ int rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out)
     return RTM_HIP_OK;
 }
 
-int rtm_hip_set_tile(rtm_hip_engine* h, int tile_x) {
+int rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x) {
     if (int rc = check_handle(h, false)) return rc;
+    if (family != -1 && family != RTM_HIP_STEP_LDS_TILE && family != RTM_HIP_STEP_MARCH)
+        return h->fail(RTM_HIP_ERR_ARG, "step kernel family must be -1, %d or %d", RTM_HIP_STEP_LDS_TILE,
+                       RTM_HIP_STEP_MARCH);
     if (tile_x != 0 && tile_x != 4 && tile_x != 8 && tile_x != 16)
         return h->fail(RTM_HIP_ERR_ARG, "tile width must be 0, 4, 8 or 16");
+    h->family_user = family;
     h->tile_x_user = tile_x;
     return RTM_HIP_OK;
 }

@@ -24,13 +24,25 @@
 // inject-then-damp.  Duplicate receiver cells add in ascending receiver order
 // through a per-tile entry table, which is the reference's sequential order.
 //
-// Mapping.  A 256-thread workgroup (4 x wave64) owns a tile of 64 depth cells
-// x tile_x columns.  The tile of p^n plus an H-cell halo is staged in LDS with
-// 256-byte coalesced row loads (the device pitch ldz is a multiple of 64
-// floats).  Each lane keeps its column's tile_x/4 + 2H values of p^n in
-// registers, so x-neighbours cost no LDS traffic; z-neighbours are LDS reads
-// of adjacent lanes (bank-conflict free).  p^{n-1}, vdt2 and p^{n+1} are
-// streamed once.  Workgroups are numbered so that each of the 8 XCDs gets a
+// Two mappings (Family in step_kernels.hpp); both keep the arithmetic above.
+//
+// kMarch (default).  Each wave64 owns one tile: a depth strip of 64
+// consecutive cells, of which the middle 64 - 2H lanes are outputs (strips
+// overlap by 2H cells), times tile_x columns.  A lane loads its depth row of
+// p^n over the tile_x + 2H columns the tile needs, and p^{n-1} and vdt2 over
+// the tile_x output columns, all before any arithmetic: tile_x*3 + 2H
+// independent 256-byte row loads in flight per wave, no LDS, no barrier.
+// x-neighbours are in the lane's own registers; z-neighbours come from the
+// lanes k above/below through cross-lane permutes.  The step is bound by how
+// many bytes are in flight, not by instruction issue, so this layout trades a
+// 2H/64 depth overlap and a 2H/tile_x column overlap (both re-read from L2)
+// for one memory round trip per wave.
+//
+// kLdsTile.  A 256-thread workgroup (4 x wave64) owns 64 depth cells x tile_x
+// columns; the tile of p^n plus its halo is staged in LDS with 256-byte row
+// loads (ldz is a multiple of 64 floats) and read back for both directions.
+//
+// In both, workgroups are numbered so that each of the 8 XCDs gets a
 // contiguous band of tiles, keeping halo re-reads inside one XCD's L2.
 #include "step_kernels.hpp"
 
@@ -137,37 +149,158 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     }
 }
 
+// Fused step, march mapping: one wave64 = one tile of (64 - 2H) x TX outputs.
+template <int H, int TX, Dir D>
+__global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
+    constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
+    constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
+
+    const int wave = (int)threadIdx.x / kTileZ;
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
+    const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
+    if (t >= a.ntiles) return;               // whole wave leaves together
+    const int tz = t % a.ntz, tx = t / a.ntz;
+    const int z = tz * TZ + lane;            // depth cell held by this lane
+    const int x0 = H + tx * TX;              // first output column
+    const bool z_load = z < a.nze;
+    const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
+
+    // ---- every load of the tile, issued before any use ------------------
+    float q[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        const int x = x0 - H + j;
+        q[j] = (z_load && x < a.nxe) ? a.p_cur[(ptrdiff_t)x * a.ldz + z] : 0.0f;
+    }
+    float pv[TX], vv[TX], wxv[TX];
+#pragma unroll
+    for (int r = 0; r < TX; ++r) {
+        const int x = x0 + r;
+        const bool ok = z_out && x < a.nxe - H;
+        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
+        pv[r] = ok ? a.p_prev[i] : 0.0f;
+        vv[r] = ok ? a.vdt2[i] : 0.0f;
+        wxv[r] = x < a.nxe ? a.wx[x] : 0.0f;
+    }
+    const float wzv = z_load ? a.wz[z] : 0.0f;
+    const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+
+#pragma unroll
+    for (int r = 0; r < TX; ++r) {
+        const float pc = q[r + H];
+        float lap = a.coef.c0 * pc;
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            const float zp = __shfl_down(pc, (unsigned)k, kTileZ);   // p^n at z + k
+            const float zm = __shfl_up(pc, (unsigned)k, kTileZ);     // p^n at z - k
+            const float tx_k = a.coef.cx[k] * (q[r + H + k] + q[r + H - k]);
+            const float tz_k = a.coef.cz[k] * (zp + zm);
+            lap = lap + (tx_k + tz_k);
+        }
+        const int x = x0 + r;
+        if (!z_out || x >= a.nxe - H) continue;
+
+        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
+        const float w = wxv[r] * wzv;
+        const float prev = pv[r] * w;
+        float out = (2.0f * pc - prev) + vv[r] * lap;
+        out = out * w;
+
+        for (int e = e0; e < e1; ++e) {
+            const Entry en = a.entries[e];
+            if (en.cell != (int)i) continue;
+            if constexpr (D == Dir::Forward) {
+                if (en.code < 0) out = out + vv[r] * a.wavelet[a.it];
+                else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
+            } else {
+                out = out + vv[r] * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
+            }
+        }
+        a.p_next[i] = out;
+
+        const int jx = x - a.nb, jz = z - a.nb;
+        if (jx >= 0 && jx < a.nx && jz >= 0 && jz < a.nz) {
+            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+            if constexpr (D == Dir::Forward) {
+                if (a.snap_out) a.snap_out[j] = out;
+            } else {
+                if (a.snap_in) {
+                    const float f = a.snap_in[j];
+                    a.image[j] = a.image[j] + f * out;
+                    a.illum[j] = a.illum[j] + f * f;
+                }
+            }
+        }
+    }
+}
+
 template <int H, Dir D>
-hipError_t launch_h(int tile_x, const StepArgs& a, hipStream_t s) {
+hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s) {
     const dim3 block(kBlockThreads);
-    const dim3 grid((unsigned)a.ntiles);
-    switch (tile_x) {
-        case 4:  hipLaunchKernelGGL((k_step<H, 1, D>), grid, block, 0, s, a); break;
-        case 8:  hipLaunchKernelGGL((k_step<H, 2, D>), grid, block, 0, s, a); break;
-        case 16: hipLaunchKernelGGL((k_step<H, 4, D>), grid, block, 0, s, a); break;
-        default: return hipErrorInvalidValue;
+    const dim3 grid((unsigned)g.blocks);
+    if (g.family == Family::kLdsTile) {
+        switch (g.tile_x) {
+            case 4:  hipLaunchKernelGGL((k_step<H, 1, D>), grid, block, 0, s, a); break;
+            case 8:  hipLaunchKernelGGL((k_step<H, 2, D>), grid, block, 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_step<H, 4, D>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (g.tile_x) {
+            case 4:  hipLaunchKernelGGL((k_march<H, 4, D>), grid, block, 0, s, a); break;
+            case 8:  hipLaunchKernelGGL((k_march<H, 8, D>), grid, block, 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_march<H, 16, D>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
 
 template <Dir D>
-hipError_t launch_dir(int half, int tile_x, const StepArgs& a, hipStream_t s) {
+hipError_t launch_dir(int half, const StepGeometry& g, const StepArgs& a, hipStream_t s) {
     switch (half) {
-        case 1: return launch_h<1, D>(tile_x, a, s);
-        case 2: return launch_h<2, D>(tile_x, a, s);
-        case 3: return launch_h<3, D>(tile_x, a, s);
-        case 4: return launch_h<4, D>(tile_x, a, s);
-        case 5: return launch_h<5, D>(tile_x, a, s);
-        case 6: return launch_h<6, D>(tile_x, a, s);
+        case 1: return launch_h<1, D>(g, a, s);
+        case 2: return launch_h<2, D>(g, a, s);
+        case 3: return launch_h<3, D>(g, a, s);
+        case 4: return launch_h<4, D>(g, a, s);
+        case 5: return launch_h<5, D>(g, a, s);
+        case 6: return launch_h<6, D>(g, a, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_step(Dir dir, int half, int tile_x, const StepArgs& a, hipStream_t stream) {
-    return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, tile_x, a, stream)
-                               : launch_dir<Dir::Backward>(half, tile_x, a, stream);
+StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz) {
+    StepGeometry g;
+    g.family = family;
+    g.tile_x = tile_x;
+    if (family == Family::kLdsTile) {
+        g.tile_z = kTileZ;
+        g.x_origin = 0;
+        g.z_origin = 0;
+        g.ntz = ldz / kTileZ;
+        g.ntx = (nxe + tile_x - 1) / tile_x;
+        g.ntiles = g.ntz * g.ntx;
+        g.blocks = g.ntiles;
+    } else {
+        // output cells [H, nze-H) x [H, nxe-H); strip tz covers depth
+        // [H + tz*(64-2H), ...) and loads [tz*(64-2H), tz*(64-2H) + 64)
+        g.tile_z = kTileZ - 2 * half;
+        g.x_origin = half;
+        g.z_origin = half;
+        g.ntz = (nze - 2 * half + g.tile_z - 1) / g.tile_z;
+        g.ntx = (nxe - 2 * half + tile_x - 1) / tile_x;
+        g.ntiles = g.ntz * g.ntx;
+        g.blocks = (g.ntiles + kWaves - 1) / kWaves;
+    }
+    return g;
+}
+
+hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream) {
+    return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, g, a, stream)
+                               : launch_dir<Dir::Backward>(half, g, a, stream);
 }
 
 }  // namespace rtm_hip

@@ -57,11 +57,34 @@ struct StepArgs {
     float*       illum;
 };
 
-// Launch one fused step on `stream`; tile_x in {4, 8, 16}.
-hipError_t launch_step(Dir dir, int half, int tile_x, const StepArgs& a, hipStream_t stream);
+// Kernel families of the fused step.
+//   kLdsTile: a 256-thread workgroup stages a 64 x tile_x tile of p^n (+halo)
+//             in LDS; tiles start at cell (0, 0) and cover the padded grid.
+//   kMarch:   each wave64 owns a depth strip of 64 cells whose 64 - 2H middle
+//             lanes are outputs (strips overlap by 2H), and tile_x columns;
+//             it loads every operand of its tile up front (no LDS, no
+//             barrier) and takes depth neighbours from other lanes.
+enum class Family : int { kLdsTile = 0, kMarch = 1 };
 
-// Number of workgroups of a step launch (one per 64 x tile_x tile).
-inline int step_tiles_z(int ldz) { return ldz / kTileZ; }
-inline int step_tiles_x(int nxe, int tile_x) { return (nxe + tile_x - 1) / tile_x; }
+// Tile grid of one launch.  Tiles are numbered x-major over (ntz x ntx):
+// tile = tx * ntz + tz; the injection table is indexed the same way.
+struct StepGeometry {
+    Family family = Family::kMarch;
+    int tile_x = 16;     // output columns per tile
+    int tile_z = 64;     // output depth cells per tile
+    int x_origin = 0;    // first column of tile 0
+    int z_origin = 0;    // first depth cell of tile 0
+    int ntz = 0, ntx = 0, ntiles = 0;
+    int blocks = 0;      // workgroups launched
+};
+
+StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz);
+
+inline int tile_of_cell(const StepGeometry& g, int ixe, int ize) {
+    return ((ixe - g.x_origin) / g.tile_x) * g.ntz + (ize - g.z_origin) / g.tile_z;
+}
+
+// Launch one fused step on `stream` with the geometry from step_geometry.
+hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream);
 
 }  // namespace rtm_hip

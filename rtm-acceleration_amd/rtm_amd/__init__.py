@@ -13,11 +13,11 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32
+from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32, STEP_LDS_TILE, STEP_MARCH
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
 __all__ = ["HipEngine", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
-           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "device_count", "mute_direct_wave",
+           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "device_count", "mute_direct_wave",
            "library_path"]
 
 
@@ -160,5 +160,7 @@ class HipEngine:
         return {"launches": list(k.launches), "sampled": list(k.sampled),
                 "sampled_ms": list(k.sampled_ms)}
 
-    def set_tile(self, tile_x: int):
-        self._check(self.lib.rtm_hip_set_tile(self.h, tile_x))
+    def set_step_kernel(self, family: int = -1, tile_x: int = 0):
+        """Step-kernel mapping (STEP_MARCH / STEP_LDS_TILE, tile width 4/8/16);
+        -1 / 0 select the built-in choice.  Results do not depend on it."""
+        self._check(self.lib.rtm_hip_set_step_kernel(self.h, family, tile_x))

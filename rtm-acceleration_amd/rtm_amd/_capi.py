@@ -15,6 +15,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "librtm_hip.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
+STEP_LDS_TILE, STEP_MARCH = 0, 1
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 
 
@@ -45,7 +46,7 @@ class Info(C.Structure):
                 ("nxe", C.c_int), ("nze", C.c_int), ("ldz", C.c_int), ("nt", C.c_int),
                 ("store_interval", C.c_int), ("num_snapshots", C.c_int),
                 ("snapshot_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-                ("snapshot_format", C.c_int), ("tile_x", C.c_int),
+                ("snapshot_format", C.c_int), ("step_family", C.c_int), ("tile_x", C.c_int),
                 ("device_name", C.c_char * 128)]
 
 
@@ -74,7 +75,7 @@ SIGNATURES = {
     "rtm_hip_reset_times": (C.c_int, [_H]),
     "rtm_hip_set_kernel_timing": (C.c_int, [_H, C.c_int]),
     "rtm_hip_get_kernel_stats": (C.c_int, [_H, C.POINTER(KernelStats)]),
-    "rtm_hip_set_tile": (C.c_int, [_H, C.c_int]),
+    "rtm_hip_set_step_kernel": (C.c_int, [_H, C.c_int, C.c_int]),
     "rtm_hip_mute_direct_wave": (None, [C.POINTER(Shot), _f32p, C.c_float, C.c_float, C.c_float]),
 }
 

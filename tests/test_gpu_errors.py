@@ -23,7 +23,9 @@ def test_setup_validation(hip_engine_cls):
         with pytest.raises(RuntimeError, match="snapshot format"):
             eng.setup(m, rtm_amd.RTMParams(nb=20, order=8), 10, 0.001, snapshot_format=9)
         with pytest.raises(RuntimeError, match="tile width"):
-            eng.set_tile(5)
+            eng.set_step_kernel(1, 5)
+        with pytest.raises(RuntimeError, match="family"):
+            eng.set_step_kernel(7, 8)
 
 
 def test_shot_validation(hip_engine_cls):

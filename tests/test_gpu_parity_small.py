@@ -14,7 +14,7 @@ import oracle_lib as O
 pytestmark = pytest.mark.gpu
 
 
-def _case(order, store, nt=160, dt=0.001, tile=0):
+def _case(order, store, nt=160, dt=0.001):
     model = small_model()
     import rtm_amd
     params = rtm_amd.RTMParams(nb=max(order // 2 + 1, 12), order=order, store_interval=store, f0=15.0)
@@ -35,10 +35,13 @@ def test_model_shot_bit_identical(hip_engine_cls, order):
             assert bits_equal(ref, got), f"max diff {np.abs(ref - got).max()}"
 
 
-@pytest.mark.parametrize("order,store,tile", [(2, 1, 4), (4, 3, 8), (6, 2, 16), (8, 2, 0),
-                                              (10, 5, 4), (12, 1, 8), (8, 7, 16)])
-def test_migrate_bit_identical(hip_engine_cls, order, store, tile):
-    print(f"case: migrate order={order} store={store} tile={tile}", flush=True)
+KERNELS = [(-1, 0), (0, 4), (0, 8), (0, 16), (1, 4), (1, 8), (1, 16)]   # (family, tile_x)
+
+
+@pytest.mark.parametrize("family,tile", KERNELS)
+@pytest.mark.parametrize("order,store", [(2, 1), (4, 3), (6, 2), (8, 2), (10, 5), (12, 1), (8, 7)])
+def test_migrate_bit_identical(hip_engine_cls, order, store, family, tile):
+    print(f"case: migrate order={order} store={store} family={family} tile={tile}", flush=True)
     model, params, shots = _case(order, store)
     orc = O.Oracle(model, params, shots[0].nt, shots[0].dt)
     for s in shots:
@@ -46,7 +49,7 @@ def test_migrate_bit_identical(hip_engine_cls, order, store, tile):
     I_ref, S_ref = orc.migrate(shots)
     with hip_engine_cls() as eng:
         eng.setup(model, params, shots[0].nt, shots[0].dt)
-        eng.set_tile(tile)
+        eng.set_step_kernel(family, tile)
         I, S = eng.migrate(shots)
     assert bits_equal(I_ref, I), f"image max diff {np.abs(I_ref - I).max()} of {np.abs(I_ref).max()}"
     assert bits_equal(S_ref, S)
