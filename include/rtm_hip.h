@@ -38,6 +38,7 @@ extern "C" {
 /* Step-kernel families (rtm_hip_set_step_kernel). */
 #define RTM_HIP_STEP_LDS_TILE  0
 #define RTM_HIP_STEP_MARCH     1
+#define RTM_HIP_STEP_MARCH_DPP 2
 
 /* Snapshot storage formats (rtm_hip_setup's params->snapshot_format). */
 #define RTM_HIP_SNAP_F32   0      /* the reference's format (src/rtm/forward.cpp:83-88) */
@@ -171,9 +172,10 @@ int         rtm_hip_reset_times(rtm_hip_engine* h);
 int         rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every);
 int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out);
 
-/* Step-kernel mapping: family RTM_HIP_STEP_MARCH (one wave64 per depth strip,
- * all loads up front) or RTM_HIP_STEP_LDS_TILE (LDS-staged 64-cell tiles), or
- * -1 for the built-in choice; tile_x = output columns per tile (4, 8 or 16),
+/* Step-kernel mapping: family RTM_HIP_STEP_MARCH / RTM_HIP_STEP_MARCH_DPP (one
+ * wave64 per depth strip, all loads up front; depth neighbours by ds_bpermute
+ * or by DPP lane shifts) or RTM_HIP_STEP_LDS_TILE (LDS-staged 64-cell tiles),
+ * or -1 for the built-in choice; tile_x = output columns per tile (4, 8 or 16),
  * 0 for the built-in choice.  Results are identical for every choice; takes
  * effect at the next forward/backward.  No reference counterpart (the CUDA
  * block shape is fixed, src/cuda/propagation_cuda.cu:96-97). */

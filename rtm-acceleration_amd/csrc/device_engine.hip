@@ -697,9 +697,9 @@ int rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out)
 
 int rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x) {
     if (int rc = check_handle(h, false)) return rc;
-    if (family != -1 && family != RTM_HIP_STEP_LDS_TILE && family != RTM_HIP_STEP_MARCH)
-        return h->fail(RTM_HIP_ERR_ARG, "step kernel family must be -1, %d or %d", RTM_HIP_STEP_LDS_TILE,
-                       RTM_HIP_STEP_MARCH);
+    if (family < -1 || family > RTM_HIP_STEP_MARCH_DPP)
+        return h->fail(RTM_HIP_ERR_ARG, "step kernel family must be -1 or %d..%d", RTM_HIP_STEP_LDS_TILE,
+                       RTM_HIP_STEP_MARCH_DPP);
     if (tile_x != 0 && tile_x != 4 && tile_x != 8 && tile_x != 16)
         return h->fail(RTM_HIP_ERR_ARG, "tile width must be 0, 4, 8 or 16");
     h->family_user = family;

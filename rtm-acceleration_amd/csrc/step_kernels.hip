@@ -71,6 +71,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int wave = (int)threadIdx.x / kTileZ;
 
+    const int iz = z0 + lane;
+    const bool z_ok = iz >= H && iz < a.nze - H;
+    const int lx0 = wave * RX;               // first local column of this wave
+
+    // ---- the streamed operands first, so their latency overlaps the staging
+    float pv[RX], vv[RX], wxv[RX];
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+        const int ix = x0 + lx0 + r;
+        const bool ok = z_ok && ix >= H && ix < a.nxe - H;
+        const ptrdiff_t i = (ptrdiff_t)(ok ? ix : 0) * a.ldz + iz;
+        pv[r] = ok ? a.p_prev[i] : 0.0f;
+        vv[r] = ok ? a.vdt2[i] : 0.0f;
+        wxv[r] = ok ? a.wx[ix] : 0.0f;
+    }
+
     // ---- stage p^n (tile + halo) in LDS; out-of-grid cells read as 0 -------
     for (int r = wave; r < LX; r += kWaves) {
         const int gx = x0 - H + r;
@@ -85,10 +101,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         }
     }
     __syncthreads();
-
-    const int iz = z0 + lane;
-    const bool z_ok = iz >= H && iz < a.nze - H;
-    const int lx0 = wave * RX;               // first local column of this wave
 
     // this lane's depth row of p^n across its columns and the x-halo
     float q[RX + 2 * H];
@@ -114,9 +126,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         }
 
         const ptrdiff_t i = (ptrdiff_t)ix * a.ldz + iz;
-        const float w = a.wx[ix] * wzv;
-        const float v2 = a.vdt2[i];
-        const float prev = a.p_prev[i] * w;
+        const float w = wxv[r] * wzv;
+        const float v2 = vv[r];
+        const float prev = pv[r] * w;
         float out = (2.0f * pc - prev) + v2 * lap;
         out = out * w;
 
@@ -149,8 +161,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     }
 }
 
+// Value held by lane+1 / lane-1 of the wave (DPP wave_shl:1 / wave_shr:1, a
+// VALU move instead of an LDS-crossbar permute).  Lanes with no source read 0;
+// they are never output lanes for shifts up to H.
+__device__ __forceinline__ float from_lane_above(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_lane_below(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
 // Fused step, march mapping: one wave64 = one tile of (64 - 2H) x TX outputs.
-template <int H, int TX, Dir D>
+// DPP selects how depth neighbours cross lanes: chained one-lane DPP shifts,
+// or ds_bpermute (__shfl_up/__shfl_down) by k lanes.
+template <int H, int TX, Dir D, bool DPP>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
@@ -190,10 +214,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     for (int r = 0; r < TX; ++r) {
         const float pc = q[r + H];
         float lap = a.coef.c0 * pc;
+        float zp = pc, zm = pc;
 #pragma unroll
         for (int k = 1; k <= H; ++k) {
-            const float zp = __shfl_down(pc, (unsigned)k, kTileZ);   // p^n at z + k
-            const float zm = __shfl_up(pc, (unsigned)k, kTileZ);     // p^n at z - k
+            if constexpr (DPP) {
+                zp = from_lane_above(zp);                           // p^n at z + k
+                zm = from_lane_below(zm);                           // p^n at z - k
+            } else {
+                zp = __shfl_down(pc, (unsigned)k, kTileZ);
+                zm = __shfl_up(pc, (unsigned)k, kTileZ);
+            }
             const float tx_k = a.coef.cx[k] * (q[r + H + k] + q[r + H - k]);
             const float tz_k = a.coef.cz[k] * (zp + zm);
             lap = lap + (tx_k + tz_k);
@@ -246,11 +276,18 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s) {
             case 16: hipLaunchKernelGGL((k_step<H, 4, D>), grid, block, 0, s, a); break;
             default: return hipErrorInvalidValue;
         }
+    } else if (g.family == Family::kMarch) {
+        switch (g.tile_x) {
+            case 4:  hipLaunchKernelGGL((k_march<H, 4, D, false>), grid, block, 0, s, a); break;
+            case 8:  hipLaunchKernelGGL((k_march<H, 8, D, false>), grid, block, 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_march<H, 16, D, false>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
         switch (g.tile_x) {
-            case 4:  hipLaunchKernelGGL((k_march<H, 4, D>), grid, block, 0, s, a); break;
-            case 8:  hipLaunchKernelGGL((k_march<H, 8, D>), grid, block, 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_march<H, 16, D>), grid, block, 0, s, a); break;
+            case 4:  hipLaunchKernelGGL((k_march<H, 4, D, true>), grid, block, 0, s, a); break;
+            case 8:  hipLaunchKernelGGL((k_march<H, 8, D, true>), grid, block, 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_march<H, 16, D, true>), grid, block, 0, s, a); break;
             default: return hipErrorInvalidValue;
         }
     }

@@ -63,8 +63,9 @@ struct StepArgs {
 //   kMarch:   each wave64 owns a depth strip of 64 cells whose 64 - 2H middle
 //             lanes are outputs (strips overlap by 2H), and tile_x columns;
 //             it loads every operand of its tile up front (no LDS, no
-//             barrier) and takes depth neighbours from other lanes.
-enum class Family : int { kLdsTile = 0, kMarch = 1 };
+//             barrier) and takes depth neighbours from other lanes through
+//             ds_bpermute (kMarch) or chained DPP row shifts (kMarchDpp).
+enum class Family : int { kLdsTile = 0, kMarch = 1, kMarchDpp = 2 };
 
 // Tile grid of one launch.  Tiles are numbered x-major over (ntz x ntx):
 // tile = tx * ntz + tz; the injection table is indexed the same way.

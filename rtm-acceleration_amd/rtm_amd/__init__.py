@@ -13,11 +13,11 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32, STEP_LDS_TILE, STEP_MARCH
+from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32, STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
 __all__ = ["HipEngine", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
-           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "device_count", "mute_direct_wave",
+           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave",
            "library_path"]
 
 

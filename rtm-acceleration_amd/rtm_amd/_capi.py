@@ -15,7 +15,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "librtm_hip.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
-STEP_LDS_TILE, STEP_MARCH = 0, 1
+STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP = 0, 1, 2
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 
 
