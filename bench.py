@@ -160,15 +160,18 @@ def main():
         "data": "synthetic (seeded faulted-layer model; traces modelled on the GPU)",
         "config": {"workload": "2301x751 @4 m, nt 8000, dt 0.35 ms, order 8, nb 60, store 4, "
                                "767 receivers, plain per-shot engine, FP32 snapshots",
-                   "engine": "hip", "step_family": ["lds-tile", "march"][info["step_family"]],
-                   "tile_x": info["tile_x"], "ldz": info["ldz"]},
+                   "engine": "hip",
+                   "step_kernel": {d: {"family": ["lds-tile", "march", "march-dpp"][info["step_family"][k]],
+                                       "tile_x": info["tile_x"][k], "autotuned": bool(info["autotuned"][k])}
+                                   for k, d in enumerate(("forward", "backward"))},
+                   "ldz": info["ldz"]},
         "shots_per_s": round(K / elapsed, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
         "whole_shot_hbm_frac": round(shot_bytes_rate / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step<H=4, Backward> (fused step + imaging)",
+                     "kernel": "backward fused step (stencil + sponge + injection + imaging), order 8",
                      "bytes_per_launch": round(bwd_b), "mean_launch_us": round(bwd_ms * 1e3, 3),
                      "forward_kernel_us": round(fwd_ms * 1e3, 3),
                      "sampled_launches": ks["sampled"]},

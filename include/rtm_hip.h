@@ -87,8 +87,9 @@ typedef struct {
     size_t snapshot_bytes;       /* device bytes of the snapshot store   */
     size_t device_bytes;         /* all device buffers of this handle    */
     int    snapshot_format;
-    int    step_family;          /* RTM_HIP_STEP_* of the step kernel    */
-    int    tile_x;               /* step-kernel tile width along x       */
+    int    step_family[2];       /* RTM_HIP_STEP_* of the forward/backward step kernel */
+    int    tile_x[2];            /* their tile widths along x            */
+    int    autotuned[2];         /* 1 when chosen by the autotuner       */
     char   device_name[128];
 } rtm_hip_info;
 
@@ -176,8 +177,10 @@ int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_sta
  * wave64 per depth strip, all loads up front; depth neighbours by ds_bpermute
  * or by DPP lane shifts) or RTM_HIP_STEP_LDS_TILE (LDS-staged 64-cell tiles),
  * or -1 for the built-in choice; tile_x = output columns per tile (4, 8 or 16),
- * 0 for the built-in choice.  Results are identical for every choice; takes
- * effect at the next forward/backward.  No reference counterpart (the CUDA
+ * 0 for the built-in choice.  The built-in choice is an autotuner that times
+ * every family and width on the engine's grid before the first propagation
+ * of each direction after setup.  Results are identical for every choice;
+ * takes effect at the next forward/backward.  No reference counterpart (the CUDA
  * block shape is fixed, src/cuda/propagation_cuda.cu:96-97). */
 int         rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x);
 

@@ -56,6 +56,8 @@ struct rtm_hip_engine {
     rtm_hip::Coeffs coef{};
     int family_user = -1;     // -1: built-in choice
     int tile_x_user = 0;      // 0: built-in choice
+    bool tuned[2] = {false, false};            // per direction: forward, backward
+    rtm_hip::StepGeometry tuned_geom[2];
 
     // device buffers
     float* d_vdt2 = nullptr;
@@ -166,16 +168,14 @@ struct rtm_hip_engine {
         ready = false;
     }
 
-    // Step-kernel tiling: the march mapping with 16-column tiles unless the
-    // caller chose otherwise; small grids take narrower tiles so the launch
-    // still spreads over every CU.
-    rtm_hip::StepGeometry geometry() const {
-        const auto fam = family_user < 0 ? rtm_hip::Family::kMarch : static_cast<rtm_hip::Family>(family_user);
+    // Step-kernel tiling of direction k (0 forward, 1 backward): the caller's
+    // choice, else the autotuned one, else the DPP march mapping with 8-column
+    // tiles (4 on small grids, so the launch still spreads over every CU).
+    rtm_hip::StepGeometry geometry(int k) const {
+        if (family_user < 0 && tile_x_user == 0 && tuned[k]) return tuned_geom[k];
+        const auto fam = family_user < 0 ? rtm_hip::Family::kMarchDpp : static_cast<rtm_hip::Family>(family_user);
         int tx = tile_x_user;
-        if (tx == 0) {
-            const long long cells = (long long)nxe * ldz;
-            tx = cells >= 1024LL * 1024 ? 16 : (cells >= 256LL * 1024 ? 8 : 4);
-        }
+        if (tx == 0) tx = (long long)nxe * ldz >= 4LL * 1024 * 1024 ? 8 : 4;
         return rtm_hip::step_geometry(fam, half, tx, nxe, nze, ldz);
     }
 };
@@ -350,16 +350,10 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
 
         const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
         EventPair ev;
-        if (sample) {
-            ev = take_event(h);
-            (void)hipEventRecord(ev.start, h->stream);
-        }
-        hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream);
+        if (sample) ev = take_event(h);
+        hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream, ev.start, ev.stop);
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
-        if (sample) {
-            (void)hipEventRecord(ev.stop, h->stream);
-            h->ev_pending.push_back({k, ev});
-        }
+        if (sample) h->ev_pending.push_back({k, ev});
         h->launches[k] += 1;
         // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
         const int old = ia;
@@ -377,10 +371,86 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     return RTM_HIP_OK;
 }
 
+// Autotuner: times every step-kernel family and tile width on this grid for
+// one direction (16 launches each, no injection entries, imaging or snapshot
+// traffic on every store-th launch as in the real loop) and keeps the
+// fastest.  Runs once per setup and direction, before the direction's first
+// propagation; the wavefields are reset afterwards by the propagation itself.
+int autotune(rtm_hip_engine* h, Dir dir) {
+    const int k = dir == Dir::Forward ? 0 : 1;
+    if (h->tuned[k] || h->family_user >= 0 || h->tile_x_user != 0) return RTM_HIP_OK;
+    const rtm_hip::Family fams[] = {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kMarch,
+                                    rtm_hip::Family::kLdsTile};
+    const int tiles[] = {4, 8, 16};
+    int max_tiles = 0;
+    for (auto f : fams)
+        for (int t : tiles)
+            max_tiles = std::max(max_tiles, rtm_hip::step_geometry(f, h->half, t, h->nxe, h->nze, h->ldz).ntiles);
+    int* d_ptr = nullptr;
+    float* d_img = nullptr;
+    if (hipMalloc(&d_ptr, (size_t)(max_tiles + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_img, 2 * h->n_int * sizeof(float)) != hipSuccess ||
+        hipMemsetAsync(d_ptr, 0, (size_t)(max_tiles + 1) * sizeof(int), h->stream) != hipSuccess ||
+        hipMemsetAsync(d_img, 0, 2 * h->n_int * sizeof(float), h->stream) != hipSuccess) {
+        if (d_ptr) (void)hipFree(d_ptr);
+        if (d_img) (void)hipFree(d_img);
+        return h->fail(RTM_HIP_ERR_MEMORY, "autotune scratch allocation failed");
+    }
+    EventPair ev = take_event(h);
+    rtm_hip::StepArgs a{};
+    a.vdt2 = h->d_vdt2; a.wx = h->d_wx; a.wz = h->d_wz; a.coef = h->coef;
+    a.nxe = h->nxe; a.nze = h->nze; a.ldz = h->ldz; a.nb = h->nb; a.nx = h->nx; a.nz = h->nz;
+    a.tile_ptr = d_ptr; a.entries = nullptr; a.wavelet = h->d_wavelet;
+    a.image = d_img; a.illum = d_img + h->n_int;
+    float best_ms = 1e30f;
+    rtm_hip::StepGeometry best = h->geometry(k);
+    int rc = RTM_HIP_OK;
+    constexpr int kWarm = 3, kTimed = 16;
+    for (auto f : fams) {
+        for (int t : tiles) {
+            const rtm_hip::StepGeometry g = rtm_hip::step_geometry(f, h->half, t, h->nxe, h->nze, h->ldz);
+            a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
+            for (int n = 0; n < kWarm + kTimed && !rc; ++n) {
+                if (n == kWarm) (void)hipEventRecord(ev.start, h->stream);
+                a.p_prev = h->d_fld[n % 3];
+                a.p_cur = h->d_fld[(n + 1) % 3];
+                a.p_next = h->d_fld[(n + 2) % 3];
+                float* slot = (n % h->store) == 0 ? h->d_snap : nullptr;
+                a.snap_out = dir == Dir::Forward ? slot : nullptr;
+                a.snap_in = dir == Dir::Backward ? slot : nullptr;
+                hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream);
+                if (e != hipSuccess) rc = h->hip_fail(e, "autotune launch");
+            }
+            (void)hipEventRecord(ev.stop, h->stream);
+            if (rc) break;
+            if (hipEventSynchronize(ev.stop) != hipSuccess) {
+                rc = h->fail(RTM_HIP_ERR_DEVICE, "autotune synchronisation failed");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev.start, ev.stop);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = g;
+            }
+        }
+        if (rc) break;
+    }
+    h->ev_pool.push_back(ev);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(d_ptr);
+    (void)hipFree(d_img);
+    if (rc) return rc;
+    h->tuned[k] = true;
+    h->tuned_geom[k] = best;
+    return RTM_HIP_OK;
+}
+
 int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out, float* recorded) {
     if (int rc = validate_shot(h, s, false)) return rc;
     rtm::Timer wall;
-    const rtm_hip::StepGeometry g = h->geometry();
+    if (int rc = autotune(h, Dir::Forward)) return rc;
+    const rtm_hip::StepGeometry g = h->geometry(0);
     const bool record = recorded != nullptr && s->nrec > 0;
     if (int rc = build_entries(h, 0, s, true, record, g)) return rc;
     if (record) {
@@ -410,7 +480,8 @@ int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out,
 int backward_impl(rtm_hip_engine* h, const rtm_hip_shot* s) {
     if (int rc = validate_shot(h, s, true)) return rc;
     rtm::Timer wall;
-    const rtm_hip::StepGeometry g = h->geometry();
+    if (int rc = autotune(h, Dir::Backward)) return rc;
+    const rtm_hip::StepGeometry g = h->geometry(1);
     if (int rc = build_entries(h, 1, s, false, true, g)) return rc;
     if (s->nrec > 0) {
         const size_t n = (size_t)s->nrec * s->nt;
@@ -528,6 +599,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     h->n_fld = (size_t)nxe * ldz;
     h->snap_bytes = (size_t)h->nsnap * h->n_int * sizeof(float);
     h->times = rtm_hip_times{};
+    h->tuned[0] = h->tuned[1] = false;
 
     rtm_hip::HostTables t;
     try {
@@ -570,6 +642,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (e == hipSuccess) e = hipMemset(h->d_image, 0, h->n_int * sizeof(float));
     if (e == hipSuccess) e = hipMemset(h->d_illum, 0, h->n_int * sizeof(float));
     if (e == hipSuccess) e = hipMemset(h->d_snap, 0, h->snap_bytes);
+    for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMemset(h->d_fld[b], 0, h->n_fld * sizeof(float));
     if (e != hipSuccess) {
         h->release();
         return h->hip_fail(e, "table upload");
@@ -590,13 +663,17 @@ int rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out) {
     out->device_bytes = h->device_bytes;
     out->snapshot_format = h->snap_fmt;
     if (h->ready) {
-        const rtm_hip::StepGeometry g = h->geometry();
-        out->tile_x = g.tile_x;
-        out->step_family = static_cast<int>(g.family);
+        for (int k = 0; k < 2; ++k) {
+            const rtm_hip::StepGeometry g = h->geometry(k);
+            out->tile_x[k] = g.tile_x;
+            out->step_family[k] = static_cast<int>(g.family);
+            out->autotuned[k] = h->tuned[k] && h->family_user < 0 && h->tile_x_user == 0;
+        }
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
-        std::snprintf(out->device_name, sizeof out->device_name, "%s (%s)", prop.name, prop.gcnArchName);
+        std::snprintf(out->device_name, sizeof out->device_name, "%s (%s)",
+                      prop.name[0] ? prop.name : "AMD Instinct", prop.gcnArchName);
     }
     return RTM_HIP_OK;
 }

@@ -46,6 +46,8 @@
 // contiguous band of tiles, keeping halo re-reads inside one XCD's L2.
 #include "step_kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace rtm_hip {
 
 namespace {
@@ -197,7 +199,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         const int x = x0 - H + j;
         q[j] = (z_load && x < a.nxe) ? a.p_cur[(ptrdiff_t)x * a.ldz + z] : 0.0f;
     }
+    const int jz = z - a.nb;                 // interior depth index
+    const bool z_int = z_out && jz >= 0 && jz < a.nz;
     float pv[TX], vv[TX], wxv[TX];
+    // backward stored steps: snapshot, image and illumination of the tile
+    constexpr int NI = D == Dir::Backward ? TX : 1;
+    float fv[NI], iv[NI], sv[NI];
+    const bool imaging = D == Dir::Backward && a.snap_in != nullptr;
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
         const int x = x0 + r;
@@ -206,6 +214,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         pv[r] = ok ? a.p_prev[i] : 0.0f;
         vv[r] = ok ? a.vdt2[i] : 0.0f;
         wxv[r] = x < a.nxe ? a.wx[x] : 0.0f;
+        if constexpr (D == Dir::Backward) {
+            const int jx = x - a.nb;
+            const bool in = imaging && ok && z_int && jx >= 0 && jx < a.nx;
+            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+            fv[r] = in ? a.snap_in[j] : 0.0f;
+            iv[r] = in ? a.image[j] : 0.0f;
+            sv[r] = in ? a.illum[j] : 0.0f;
+        }
     }
     const float wzv = z_load ? a.wz[z] : 0.0f;
     const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
@@ -249,45 +265,52 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         }
         a.p_next[i] = out;
 
-        const int jx = x - a.nb, jz = z - a.nb;
-        if (jx >= 0 && jx < a.nx && jz >= 0 && jz < a.nz) {
+        const int jx = x - a.nb;
+        if (z_int && jx >= 0 && jx < a.nx) {
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
             if constexpr (D == Dir::Forward) {
                 if (a.snap_out) a.snap_out[j] = out;
             } else {
-                if (a.snap_in) {
-                    const float f = a.snap_in[j];
-                    a.image[j] = a.image[j] + f * out;
-                    a.illum[j] = a.illum[j] + f * f;
+                if (imaging) {
+                    a.image[j] = iv[r] + fv[r] * out;
+                    a.illum[j] = sv[r] + fv[r] * fv[r];
                 }
             }
         }
     }
 }
 
+// Launches a kernel; with events, through hipExtLaunchKernelGGL so that the
+// events carry the kernel's own start and end timestamps.
+#define RTM_LAUNCH(KERNEL)                                                                   \
+    do {                                                                                     \
+        if (t0) hipExtLaunchKernelGGL((KERNEL), grid, block, 0, s, t0, t1, 0, a);            \
+        else hipLaunchKernelGGL((KERNEL), grid, block, 0, s, a);                             \
+    } while (0)
+
 template <int H, Dir D>
-hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s) {
+hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)g.blocks);
     if (g.family == Family::kLdsTile) {
         switch (g.tile_x) {
-            case 4:  hipLaunchKernelGGL((k_step<H, 1, D>), grid, block, 0, s, a); break;
-            case 8:  hipLaunchKernelGGL((k_step<H, 2, D>), grid, block, 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_step<H, 4, D>), grid, block, 0, s, a); break;
+            case 4:  RTM_LAUNCH((k_step<H, 1, D>)); break;
+            case 8:  RTM_LAUNCH((k_step<H, 2, D>)); break;
+            case 16: RTM_LAUNCH((k_step<H, 4, D>)); break;
             default: return hipErrorInvalidValue;
         }
     } else if (g.family == Family::kMarch) {
         switch (g.tile_x) {
-            case 4:  hipLaunchKernelGGL((k_march<H, 4, D, false>), grid, block, 0, s, a); break;
-            case 8:  hipLaunchKernelGGL((k_march<H, 8, D, false>), grid, block, 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_march<H, 16, D, false>), grid, block, 0, s, a); break;
+            case 4:  RTM_LAUNCH((k_march<H, 4, D, false>)); break;
+            case 8:  RTM_LAUNCH((k_march<H, 8, D, false>)); break;
+            case 16: RTM_LAUNCH((k_march<H, 16, D, false>)); break;
             default: return hipErrorInvalidValue;
         }
     } else {
         switch (g.tile_x) {
-            case 4:  hipLaunchKernelGGL((k_march<H, 4, D, true>), grid, block, 0, s, a); break;
-            case 8:  hipLaunchKernelGGL((k_march<H, 8, D, true>), grid, block, 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_march<H, 16, D, true>), grid, block, 0, s, a); break;
+            case 4:  RTM_LAUNCH((k_march<H, 4, D, true>)); break;
+            case 8:  RTM_LAUNCH((k_march<H, 8, D, true>)); break;
+            case 16: RTM_LAUNCH((k_march<H, 16, D, true>)); break;
             default: return hipErrorInvalidValue;
         }
     }
@@ -295,14 +318,15 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s) {
 }
 
 template <Dir D>
-hipError_t launch_dir(int half, const StepGeometry& g, const StepArgs& a, hipStream_t s) {
+hipError_t launch_dir(int half, const StepGeometry& g, const StepArgs& a, hipStream_t s, hipEvent_t t0,
+                      hipEvent_t t1) {
     switch (half) {
-        case 1: return launch_h<1, D>(g, a, s);
-        case 2: return launch_h<2, D>(g, a, s);
-        case 3: return launch_h<3, D>(g, a, s);
-        case 4: return launch_h<4, D>(g, a, s);
-        case 5: return launch_h<5, D>(g, a, s);
-        case 6: return launch_h<6, D>(g, a, s);
+        case 1: return launch_h<1, D>(g, a, s, t0, t1);
+        case 2: return launch_h<2, D>(g, a, s, t0, t1);
+        case 3: return launch_h<3, D>(g, a, s, t0, t1);
+        case 4: return launch_h<4, D>(g, a, s, t0, t1);
+        case 5: return launch_h<5, D>(g, a, s, t0, t1);
+        case 6: return launch_h<6, D>(g, a, s, t0, t1);
         default: return hipErrorInvalidValue;
     }
 }
@@ -335,9 +359,10 @@ StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze
     return g;
 }
 
-hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream) {
-    return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, g, a, stream)
-                               : launch_dir<Dir::Backward>(half, g, a, stream);
+hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
+                       hipEvent_t start, hipEvent_t stop) {
+    return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, g, a, stream, start, stop)
+                               : launch_dir<Dir::Backward>(half, g, a, stream, start, stop);
 }
 
 }  // namespace rtm_hip

@@ -86,6 +86,8 @@ inline int tile_of_cell(const StepGeometry& g, int ixe, int ize) {
 }
 
 // Launch one fused step on `stream` with the geometry from step_geometry.
-hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream);
+// With start/stop events, they record the kernel's own start and end.
+hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
+                       hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 }  // namespace rtm_hip

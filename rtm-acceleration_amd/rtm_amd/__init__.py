@@ -87,6 +87,8 @@ class HipEngine:
         i = _capi.Info()
         self._check(self.lib.rtm_hip_get_info(self.h, C.byref(i)))
         out = {n: getattr(i, n) for n, _ in _capi.Info._fields_}
+        for n in ("step_family", "tile_x", "autotuned"):
+            out[n] = list(out[n])
         out["device_name"] = i.device_name.decode()
         return out
 

@@ -46,7 +46,8 @@ class Info(C.Structure):
                 ("nxe", C.c_int), ("nze", C.c_int), ("ldz", C.c_int), ("nt", C.c_int),
                 ("store_interval", C.c_int), ("num_snapshots", C.c_int),
                 ("snapshot_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-                ("snapshot_format", C.c_int), ("step_family", C.c_int), ("tile_x", C.c_int),
+                ("snapshot_format", C.c_int), ("step_family", C.c_int * 2), ("tile_x", C.c_int * 2),
+                ("autotuned", C.c_int * 2),
                 ("device_name", C.c_char * 128)]
 
 
