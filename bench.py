@@ -20,7 +20,14 @@ Reported (one JSON line):
                engine's stream around sampled launches in the timed region
   cpu_baseline the reference's cpu-opt engine, compiled from its own sources
                (oracle/_ref), timed on this host on a bounded sample of the
-               same workload (same grid, fewer time steps)
+               same workload (one shot, nt capped to a ~20 s budget)
+  traffic      HBM bytes per backward-step launch from the rocprofv3 counter
+               runs summarised in profiles/step_kernel_traffic_2301x751.md
+               (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading
+               for the copy probe of known size), when that file is present
+
+--nt shortens the shot for counter-collection runs only; the reported line
+always uses the full nt = 8000 unless --nt is given.
 """
 from __future__ import annotations
 
@@ -39,11 +46,11 @@ HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, published
 METRIC = "Gcell-steps/s & % HBM roofline; migrated shots/s + p99 latency vs plain GPU"
 
 
-def workload(nshots_needed: int):
+def workload(nshots_needed: int, nt: int = 8000):
     from rtm_amd import RTMParams, models
     model = models.faulted_layers(2301, 751, 4.0, 4.0)
     params = RTMParams(nb=60, order=8, store_interval=4, f0=15.0)
-    nt, dt = 8000, 0.00035
+    dt = 0.00035
     rec_x = 12.0
     shot_x = 48.0 * np.arange(192)
     # take shots spread over the line rather than the first few
@@ -60,6 +67,18 @@ def algorithmic_bytes(model, params, nt, snap_bytes=4):
     fwd_launch = 16.0 * n_ext + nsnap * n_int * snap_bytes / nt
     bwd_launch = 16.0 * n_ext + nsnap * n_int * (snap_bytes + 16) / nt
     return n_ext, per_shot, fwd_launch, bwd_launch
+
+
+def traffic_per_launch():
+    """HBM bytes per backward-step launch from the committed counter summary
+    (a line 'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(REPO, "profiles", "step_kernel_traffic_2301x751.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
@@ -79,7 +98,7 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
         orc.migrate([shot])
         threads = O.lib().orc_num_threads(orc.h)
     probe = time.perf_counter() - t0
-    nt_sample = int(max(40, min(2000, nt_probe * budget_s / max(probe, 1e-3))))
+    nt_sample = int(max(40, min(shot.nt, nt_probe * budget_s / max(probe, 1e-3))))
     t0 = time.perf_counter()
     if kind == "reference":
         _, _, times = O.ref_migrate("cpu-opt", model, params, nt_sample, dt, [shot])
@@ -93,7 +112,7 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
     cells = 2.0 * nt_sample * nxe * nze
     return {"value": round(cells / wall * 1e-9, 4), "unit": "Gcell-steps/s", "cores": int(threads),
             "kind": kind,
-            "sample": f"cpu-opt engine, 1 shot of the 2301x751 workload, nt={nt_sample} of 8000 "
+            "sample": f"cpu-opt engine, 1 shot of the 2301x751 workload, nt={nt_sample} of {shot.nt} "
                       f"({cells * 1e-9:.2f} G cell-steps, {wall:.1f} s wall, {prop:.1f} s propagation), "
                       f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
 
@@ -107,6 +126,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="step-kernel tile width (0 = built-in)")
     ap.add_argument("--sample-every", type=int, default=32, help="event-time every N-th step launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--nt", type=int, default=8000, help="time steps per shot (8000 = the workload)")
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30, help="copy-probe buffer size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -117,7 +138,8 @@ def main():
     if rtm_amd.device_count() < 1:
         sys.exit("bench: no HIP device")
     K, W = args.steps, args.warmup
-    model, params, nt, dt, shots = workload(K + W)
+    model, params, nt, dt, shots = workload(K + W, args.nt)
+    copy_peak = rtm_amd.bandwidth_probe(0, args.probe_bytes, 5)
     n_ext, bytes_shot, fwd_b, bwd_b = algorithmic_bytes(model, params, nt)
 
     eng = rtm_amd.HipEngine(0)
@@ -170,7 +192,9 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
         "whole_shot_hbm_frac": round(shot_bytes_rate / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
+                     "copy_probe_gbs": round(copy_peak, 1),
+                     "frac_of_copy_probe": round(achieved / copy_peak, 4) if copy_peak else None,
                      "kernel": "backward fused step (stencil + sponge + injection + imaging), order 8",
                      "bytes_per_launch": round(bwd_b), "mean_launch_us": round(bwd_ms * 1e3, 3),
                      "forward_kernel_us": round(fwd_ms * 1e3, 3),

@@ -184,6 +184,12 @@ int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_sta
  * block shape is fixed, src/cuda/propagation_cuda.cu:96-97). */
 int         rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x);
 
+/* Replaces the reference's GPU bandwidth probe (src/tools/bandwidth_probe_cuda.cu:10-66):
+ * copies `bytes` between two device buffers `repeats` times (after one
+ * warm-up) and returns the best (read + write) rate in GB/s.  Allocates and
+ * frees its own buffers on `device`. */
+int         rtm_hip_bandwidth_probe(int device, size_t bytes, int repeats, double* best_gbs);
+
 /* Replaces rtm::mute_direct_wave (include/rtm_io.hpp:9; src/io/seismic_io.cpp:18-35),
  * host-only: traces (nrec*nt, stride shot->nt) muted in place. */
 void        rtm_hip_mute_direct_wave(const rtm_hip_shot* shot, float* traces,

@@ -17,7 +17,7 @@ from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32, STEP_LDS_TILE, STEP_MARCH, STE
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
 __all__ = ["HipEngine", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
-           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave",
+           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave", "bandwidth_probe",
            "library_path"]
 
 
@@ -27,6 +27,16 @@ def library_path() -> str:
 
 def device_count() -> int:
     return int(_capi.load().rtm_hip_device_count())
+
+
+def bandwidth_probe(device: int = 0, nbytes: int = 1 << 30, repeats: int = 5) -> float:
+    """Best device copy rate, GB/s (read + write bytes)."""
+    lib = _capi.load()
+    out = C.c_double()
+    rc = lib.rtm_hip_bandwidth_probe(device, nbytes, repeats, C.byref(out))
+    if rc != _capi.OK:
+        raise RuntimeError(f"bandwidth probe failed ({rc})")
+    return out.value
 
 
 def mute_direct_wave(shot: ShotRecord, v_surface: float, t_pad: float, t_taper: float) -> None:

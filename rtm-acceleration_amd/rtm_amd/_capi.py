@@ -78,6 +78,7 @@ SIGNATURES = {
     "rtm_hip_get_kernel_stats": (C.c_int, [_H, C.POINTER(KernelStats)]),
     "rtm_hip_set_step_kernel": (C.c_int, [_H, C.c_int, C.c_int]),
     "rtm_hip_mute_direct_wave": (None, [C.POINTER(Shot), _f32p, C.c_float, C.c_float, C.c_float]),
+    "rtm_hip_bandwidth_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None
