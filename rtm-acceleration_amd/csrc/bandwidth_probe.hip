@@ -1,7 +1,7 @@
 // bandwidth_probe.hip -- device copy bandwidth (the reference's GPU probe,
 // src/tools/bandwidth_probe_cuda.cu:10-66: dst[i] = src[i] over two buffers,
-// best of N).  16-byte loads and stores, a grid of 8 workgroups per CU and a
-// grid-stride loop; reported as (read + write bytes) / kernel time.
+// best of N).  One 16-byte load and store per thread over the whole buffer;
+// reported as (read + write bytes) / kernel time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,8 +10,8 @@
 
 namespace {
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
 }
 }  // namespace
 
@@ -25,8 +25,12 @@ extern "C" int rtm_hip_bandwidth_probe(int device, size_t bytes, int repeats, do
         (void)hipFree(a);
         return RTM_HIP_ERR_MEMORY;
     }
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    const size_t blocks = (n4 + 255) / 256;
+    if (blocks > 0x7fffffffULL) {
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return RTM_HIP_ERR_ARG;
+    }
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
@@ -34,7 +38,7 @@ extern "C" int rtm_hip_bandwidth_probe(int device, size_t bytes, int repeats, do
     double best = 0.0;
     for (int r = 0; r < repeats + 1 && rc == RTM_HIP_OK; ++r) {
         (void)hipEventRecord(t0, nullptr);
-        hipLaunchKernelGGL(k_copy, dim3(8 * cus), dim3(256), 0, nullptr, a, b, n4);
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, nullptr, a, b, n4);
         (void)hipEventRecord(t1, nullptr);
         if (hipEventSynchronize(t1) != hipSuccess) { rc = RTM_HIP_ERR_DEVICE; break; }
         float ms = 0.0f;

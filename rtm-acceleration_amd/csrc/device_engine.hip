@@ -372,7 +372,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
 }
 
 // Autotuner: times every step-kernel family and tile width on this grid for
-// one direction (16 launches each, no injection entries, imaging or snapshot
+// one direction (32 launches each, no injection entries, imaging or snapshot
 // traffic on every store-th launch as in the real loop) and keeps the
 // fastest.  Runs once per setup and direction, before the direction's first
 // propagation; the wavefields are reset afterwards by the propagation itself.
@@ -405,7 +405,7 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     float best_ms = 1e30f;
     rtm_hip::StepGeometry best = h->geometry(k);
     int rc = RTM_HIP_OK;
-    constexpr int kWarm = 3, kTimed = 16;
+    constexpr int kWarm = 4, kTimed = 32;
     for (auto f : fams) {
         for (int t : tiles) {
             const rtm_hip::StepGeometry g = rtm_hip::step_geometry(f, h->half, t, h->nxe, h->nze, h->ldz);
@@ -429,7 +429,9 @@ int autotune(rtm_hip_engine* h, Dir dir) {
             }
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, ev.start, ev.stop);
-            if (ms < best_ms) {
+            // candidates are tried in order of preference; a later one must
+            // win by 3 % so that timing noise does not flip the choice
+            if (ms < 0.97f * best_ms) {
                 best_ms = ms;
                 best = g;
             }
