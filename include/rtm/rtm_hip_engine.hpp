@@ -18,7 +18,9 @@ namespace rtm {
 
 class HipRTM : public RTMEngine {
 public:
-    explicit HipRTM(int device = 0, int snapshot_format = 0);
+    // snapshot_format: RTM_HIP_SNAP_*; image_stack: RTM_HIP_STACK_* (0, 0 is
+    // the reference's FP32 snapshots and FP32 running stack)
+    explicit HipRTM(int device = 0, int snapshot_format = 0, int image_stack = 0);
     ~HipRTM() override;
     HipRTM(const HipRTM&) = delete;
     HipRTM& operator=(const HipRTM&) = delete;
@@ -48,6 +50,7 @@ private:
     void pull_times();
     rtm_hip_engine* h_ = nullptr;
     int snapshot_format_ = 0;
+    int image_stack_ = 0;
 };
 
 }  // namespace rtm

@@ -45,6 +45,10 @@ extern "C" {
 #define RTM_HIP_SNAP_F16   1      /* IEEE half + one FP32 scale per snapshot            */
 #define RTM_HIP_SNAP_BF16  2      /* bfloat16 + one FP32 scale per snapshot             */
 
+/* Cross-shot image stack (rtm_hip_setup's params->image_stack). */
+#define RTM_HIP_STACK_F32  0      /* FP32 running sum over shots, the reference's (src/rtm/rtm.cpp:92-101) */
+#define RTM_HIP_STACK_F64  1      /* per-shot FP32 image added into an FP64 stack after each shot      */
+
 /* rtm::Grid, interior part (include/rtm_types.hpp:16-29). */
 typedef struct {
     int   nx, nz;
@@ -52,7 +56,7 @@ typedef struct {
     float ox, oz;
 } rtm_hip_grid;
 
-/* rtm::RTMParams (include/rtm_types.hpp:33-40) plus the snapshot format. */
+/* rtm::RTMParams (include/rtm_types.hpp:33-40) plus storage precisions. */
 typedef struct {
     int   nb;
     int   order;
@@ -60,7 +64,8 @@ typedef struct {
     int   store_interval;
     float f0;
     int   verbose;
-    int   snapshot_format;       /* RTM_HIP_SNAP_*; 0 reproduces the reference */
+    int   snapshot_format;       /* RTM_HIP_SNAP_*; F32 reproduces the reference       */
+    int   image_stack;           /* RTM_HIP_STACK_*; F32 reproduces the reference      */
 } rtm_hip_params;
 
 /* rtm::ShotRecord (include/rtm_types.hpp:49-57). */
@@ -87,6 +92,7 @@ typedef struct {
     size_t snapshot_bytes;       /* device bytes of the snapshot store   */
     size_t device_bytes;         /* all device buffers of this handle    */
     int    snapshot_format;
+    int    image_stack;
     int    step_family[2];       /* RTM_HIP_STEP_* of the forward/backward step kernel */
     int    tile_x[2];            /* their tile widths along x            */
     int    autotuned[2];         /* 1 when chosen by the autotuner       */
@@ -152,6 +158,9 @@ int         rtm_hip_backward(rtm_hip_engine* h, const rtm_hip_shot* shot);
  * (src/cuda/rtm_cuda.cu:222-244). */
 int         rtm_hip_image_zero(rtm_hip_engine* h);
 int         rtm_hip_image_get(rtm_hip_engine* h, float* image, float* illum);
+/* The FP64 stacks of an engine set up with RTM_HIP_STACK_F64 (no reference
+ * counterpart: the reference stacks in FP32). */
+int         rtm_hip_image_get_f64(rtm_hip_engine* h, double* image, double* illum);
 
 /* Replaces RTMEngine::migrate (include/rtm_engine.hpp:45-47; src/rtm/rtm.cpp:83-103;
  * src/cuda/rtm_cuda.cu:214-250): zero the image, forward + backward every shot

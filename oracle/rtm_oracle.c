@@ -3,7 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY (see rtm_oracle.h).  Built at -O2 without FP
  * contraction, like the reference's own build (CMakeLists.txt:15-17), so the
- * FP32 instance reproduces the reference's bits.
+ * FP32 instance reproduces the reference's bits.  The FP32 "cpu" instance is
+ * single-threaded like the reference's; the FP64 instance runs its cell loops
+ * on every host core (OpenMP), which changes no result.
  */
 #define _GNU_SOURCE
 #include "rtm_oracle_internal.h"

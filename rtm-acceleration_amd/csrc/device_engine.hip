@@ -65,10 +65,15 @@ struct rtm_hip_engine {
     float* d_wz = nullptr;
     float* d_wavelet = nullptr;
     float* d_fld[3] = {nullptr, nullptr, nullptr};
-    float* d_snap = nullptr;
+    float* d_snap = nullptr;         // FP32 snapshots
+    uint16_t* d_snap16 = nullptr;    // FP16/BF16 snapshot codes
+    float* d_snap_max = nullptr;     // per-snapshot max|p| (16-bit formats)
     size_t snap_bytes = 0;
     float* d_image = nullptr;
     float* d_illum = nullptr;
+    int image_stack = 0;             // 0: FP32 running sum, 1: FP64 stack
+    double* d_stack_i = nullptr;
+    double* d_stack_s = nullptr;
     int* d_tile_ptr[2] = {nullptr, nullptr};
     size_t tile_ptr_cap[2] = {0, 0};
     Entry* d_ent[2] = {nullptr, nullptr};
@@ -145,7 +150,8 @@ struct rtm_hip_engine {
         };
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
-        f(d_snap); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        f(d_snap); f(d_snap16); f(d_snap_max); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        f(d_stack_i); f(d_stack_s);
         for (int k = 0; k < 2; ++k) {
             f(d_tile_ptr[k]); f(d_ent[k]);
             tile_ptr_cap[k] = ent_cap[k] = 0;
@@ -304,6 +310,38 @@ int validate_shot(rtm_hip_engine* h, const rtm_hip_shot* s, bool need_traces) {
     return RTM_HIP_OK;
 }
 
+// Snapshot operands of the launch for time index it (see StepArgs).
+void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepArgs& a) {
+    a.snap_fmt = h->snap_fmt;
+    a.snap_out = nullptr;
+    a.snap_max = nullptr;
+    a.quant_out = nullptr;
+    a.quant_max = nullptr;
+    a.snap_in = nullptr;
+    a.snap16_in = nullptr;
+    a.snap16_max = nullptr;
+    const bool stored = (it % h->store) == 0;
+    const size_t slot = (size_t)(it / h->store);
+    if (h->snap_fmt == RTM_HIP_SNAP_F32) {
+        if (!stored) return;
+        if (dir == Dir::Forward) a.snap_out = h->d_snap + slot * h->n_int;
+        else a.snap_in = h->d_snap + slot * h->n_int;
+        return;
+    }
+    if (dir == Dir::Forward) {
+        if (stored) a.snap_max = reinterpret_cast<unsigned*>(h->d_snap_max + slot);
+        // p^n of this launch is the field of step it-1: quantise it if stored
+        if (it >= 1 && ((it - 1) % h->store) == 0) {
+            const size_t prev = (size_t)((it - 1) / h->store);
+            a.quant_out = h->d_snap16 + prev * h->n_int;
+            a.quant_max = h->d_snap_max + prev;
+        }
+    } else if (stored) {
+        a.snap16_in = h->d_snap16 + slot * h->n_int;
+        a.snap16_max = h->d_snap_max + slot;
+    }
+}
+
 // Runs one propagation (forward or backward) of h->nt steps.
 int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
              const rtm_hip::StepGeometry& g) {
@@ -331,6 +369,10 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         hipError_t e = hipMemsetAsync(p, 0, h->n_fld * sizeof(float), h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "wavefield reset");
     }
+    if (dir == Dir::Forward && h->d_snap_max) {
+        hipError_t e = hipMemsetAsync(h->d_snap_max, 0, (size_t)h->nsnap * sizeof(float), h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "snapshot scale reset");
+    }
     if (!h->loop_ev.start) {
         (void)hipEventCreate(&h->loop_ev.start);
         (void)hipEventCreate(&h->loop_ev.stop);
@@ -344,9 +386,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         a.p_prev = h->d_fld[ia];
         a.p_cur = h->d_fld[ib];
         a.p_next = h->d_fld[ic];
-        float* slot = (it % h->store) == 0 ? h->d_snap + (size_t)(it / h->store) * h->n_int : nullptr;
-        a.snap_out = dir == Dir::Forward ? slot : nullptr;
-        a.snap_in = dir == Dir::Backward ? slot : nullptr;
+        set_snapshot_args(h, dir, it, a);
 
         const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
         EventPair ev;
@@ -360,6 +400,19 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         ia = ib;
         ib = ic;
         ic = old;
+    }
+    // a 16-bit snapshot of the last step has no following step to quantise it
+    if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && ((h->nt - 1) % h->store) == 0) {
+        const size_t slot = (size_t)((h->nt - 1) / h->store);
+        hipError_t e = rtm_hip::launch_quantise(h->d_fld[ib], h->ldz, h->nb, h->nx, h->nz,
+                                                static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
+                                                h->d_snap16 + slot * h->n_int, h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
+    }
+    if (dir == Dir::Backward && h->image_stack) {
+        hipError_t e = rtm_hip::launch_fold_fp64(h->d_image, h->d_illum, h->d_stack_i, h->d_stack_s, h->n_int,
+                                                 h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "FP64 stack update");
     }
     (void)hipEventRecord(h->loop_ev.stop, h->stream);
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -415,9 +468,9 @@ int autotune(rtm_hip_engine* h, Dir dir) {
                 a.p_prev = h->d_fld[n % 3];
                 a.p_cur = h->d_fld[(n + 1) % 3];
                 a.p_next = h->d_fld[(n + 2) % 3];
-                float* slot = (n % h->store) == 0 ? h->d_snap : nullptr;
-                a.snap_out = dir == Dir::Forward ? slot : nullptr;
-                a.snap_in = dir == Dir::Backward ? slot : nullptr;
+                // slot 0 (and its neighbour for 16-bit quantisation) stand in
+                // for the real slots; the propagation rewrites them
+                set_snapshot_args(h, dir, n % std::min(h->nt, 2 * h->store), a);
                 hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream);
                 if (e != hipSuccess) rc = h->hip_fail(e, "autotune launch");
             }
@@ -471,9 +524,25 @@ int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out,
         if (e != hipSuccess) return h->hip_fail(e, "download of recorded traces");
     }
     if (snapshots_out) {
-        hipError_t e = hipMemcpy(snapshots_out, h->d_snap, (size_t)h->nsnap * h->n_int * sizeof(float),
-                                 hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return h->hip_fail(e, "download of snapshots");
+        const size_t n = (size_t)h->nsnap * h->n_int;
+        if (h->snap_fmt == RTM_HIP_SNAP_F32) {
+            hipError_t e = hipMemcpy(snapshots_out, h->d_snap, n * sizeof(float), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return h->hip_fail(e, "download of snapshots");
+        } else {
+            std::vector<uint16_t> codes(n);
+            std::vector<float> maxv(h->nsnap);
+            hipError_t e = hipMemcpy(codes.data(), h->d_snap16, n * sizeof(uint16_t), hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                e = hipMemcpy(maxv.data(), h->d_snap_max, h->nsnap * sizeof(float), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return h->hip_fail(e, "download of snapshots");
+            const auto fmt = static_cast<rtm_hip::SnapFmt>(h->snap_fmt);
+            for (int s16 = 0; s16 < h->nsnap; ++s16) {
+                const int k = rtm_hip::snapshot_scale_exponent(maxv[s16]);
+                const size_t base = (size_t)s16 * h->n_int;
+                for (size_t j = 0; j < h->n_int; ++j)
+                    snapshots_out[base + j] = rtm_hip::decode_snapshot(codes[base + j], k, fmt);
+            }
+        }
     }
     h->times.d2h += d2h.elapsed();
     return RTM_HIP_OK;
@@ -579,9 +648,11 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (nt < 1) return h->fail(RTM_HIP_ERR_ARG, "nt must be >= 1");
     if (!(dt > 0.0f)) return h->fail(RTM_HIP_ERR_ARG, "dt must be > 0");
     if (!(g->dx > 0.0f) || !(g->dz > 0.0f)) return h->fail(RTM_HIP_ERR_ARG, "dx and dz must be > 0");
-    if (p->snapshot_format != RTM_HIP_SNAP_F32)
-        return h->fail(RTM_HIP_ERR_ARG, "snapshot format %d is not available in this build",
-                       p->snapshot_format);
+    if (p->snapshot_format != RTM_HIP_SNAP_F32 && p->snapshot_format != RTM_HIP_SNAP_F16 &&
+        p->snapshot_format != RTM_HIP_SNAP_BF16)
+        return h->fail(RTM_HIP_ERR_ARG, "unknown snapshot format %d", p->snapshot_format);
+    if (p->image_stack != RTM_HIP_STACK_F32 && p->image_stack != RTM_HIP_STACK_F64)
+        return h->fail(RTM_HIP_ERR_ARG, "unknown image stack precision %d", p->image_stack);
     const long long nxe = (long long)g->nx + 2LL * p->nb, nze = (long long)g->nz + 2LL * p->nb;
     const long long ldz = round_up((int)nze, rtm_hip::kTileZ);
     if (nxe * ldz >= (1LL << 31)) return h->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
@@ -596,10 +667,12 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     h->nxe = (int)nxe; h->nze = (int)nze; h->ldz = (int)ldz;
     h->nt = nt; h->dt = dt; h->store = p->store_interval; h->verbose = p->verbose;
     h->snap_fmt = p->snapshot_format;
+    h->image_stack = p->image_stack;
     h->nsnap = (nt - 1) / p->store_interval + 1;
     h->n_int = (size_t)g->nx * g->nz;
     h->n_fld = (size_t)nxe * ldz;
-    h->snap_bytes = (size_t)h->nsnap * h->n_int * sizeof(float);
+    h->snap_bytes = (size_t)h->nsnap * h->n_int *
+                    (h->snap_fmt == RTM_HIP_SNAP_F32 ? sizeof(float) : sizeof(uint16_t));
     h->times = rtm_hip_times{};
     h->tuned[0] = h->tuned[1] = false;
 
@@ -614,7 +687,8 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
 
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-        const size_t need = h->snap_bytes + 5 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float);
+        const size_t need = h->snap_bytes + 5 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float) +
+                            (h->image_stack ? 2 * h->n_int * sizeof(double) : 0);
         if (need > free_b)
             return h->fail(RTM_HIP_ERR_MEMORY,
                            "engine needs %.2f GiB of device memory (snapshots %.2f GiB), %.2f GiB free; "
@@ -628,7 +702,16 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (!rc) rc = h->alloc(&h->d_wz, (size_t)h->ldz, "wz");
     if (!rc) rc = h->alloc(&h->d_wavelet, (size_t)nt, "wavelet");
     for (int b = 0; b < 3 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");
-    if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
+    if (h->snap_fmt == RTM_HIP_SNAP_F32) {
+        if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
+    } else {
+        if (!rc) rc = h->alloc(&h->d_snap16, h->nsnap * h->n_int, "snapshot store");
+        if (!rc) rc = h->alloc(&h->d_snap_max, (size_t)h->nsnap, "snapshot scales");
+    }
+    if (h->image_stack) {
+        if (!rc) rc = h->alloc(&h->d_stack_i, h->n_int, "FP64 image stack");
+        if (!rc) rc = h->alloc(&h->d_stack_s, h->n_int, "FP64 illumination stack");
+    }
     if (!rc) rc = h->alloc(&h->d_image, h->n_int, "image");
     if (!rc) rc = h->alloc(&h->d_illum, h->n_int, "illumination");
     if (rc) {
@@ -643,7 +726,11 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
         e = hipMemcpy(h->d_wavelet, t.wavelet.data(), t.wavelet.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->d_image, 0, h->n_int * sizeof(float));
     if (e == hipSuccess) e = hipMemset(h->d_illum, 0, h->n_int * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(h->d_snap, 0, h->snap_bytes);
+    if (e == hipSuccess && h->d_snap) e = hipMemset(h->d_snap, 0, h->snap_bytes);
+    if (e == hipSuccess && h->d_snap16) e = hipMemset(h->d_snap16, 0, h->snap_bytes);
+    if (e == hipSuccess && h->d_snap_max) e = hipMemset(h->d_snap_max, 0, (size_t)h->nsnap * sizeof(float));
+    if (e == hipSuccess && h->d_stack_i) e = hipMemset(h->d_stack_i, 0, h->n_int * sizeof(double));
+    if (e == hipSuccess && h->d_stack_s) e = hipMemset(h->d_stack_s, 0, h->n_int * sizeof(double));
     for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMemset(h->d_fld[b], 0, h->n_fld * sizeof(float));
     if (e != hipSuccess) {
         h->release();
@@ -664,6 +751,7 @@ int rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out) {
     out->snapshot_bytes = h->snap_bytes;
     out->device_bytes = h->device_bytes;
     out->snapshot_format = h->snap_fmt;
+    out->image_stack = h->image_stack;
     if (h->ready) {
         for (int k = 0; k < 2; ++k) {
             const rtm_hip::StepGeometry g = h->geometry(k);
@@ -697,6 +785,8 @@ int rtm_hip_image_zero(rtm_hip_engine* h) {
     (void)hipSetDevice(h->device);
     hipError_t e = hipMemsetAsync(h->d_image, 0, h->n_int * sizeof(float), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->d_illum, 0, h->n_int * sizeof(float), h->stream);
+    if (e == hipSuccess && h->d_stack_i) e = hipMemsetAsync(h->d_stack_i, 0, h->n_int * sizeof(double), h->stream);
+    if (e == hipSuccess && h->d_stack_s) e = hipMemsetAsync(h->d_stack_s, 0, h->n_int * sizeof(double), h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image reset");
 }
@@ -706,10 +796,37 @@ int rtm_hip_image_get(rtm_hip_engine* h, float* image, float* illum) {
     (void)hipSetDevice(h->device);
     rtm::Timer t;
     hipError_t e = hipStreamSynchronize(h->stream);
+    if (h->image_stack) {
+        // the FP64 stacks, rounded once to float
+        std::vector<double> tmp(h->n_int);
+        if (e == hipSuccess && image) {
+            e = hipMemcpy(tmp.data(), h->d_stack_i, h->n_int * sizeof(double), hipMemcpyDeviceToHost);
+            for (size_t j = 0; e == hipSuccess && j < h->n_int; ++j) image[j] = (float)tmp[j];
+        }
+        if (e == hipSuccess && illum) {
+            e = hipMemcpy(tmp.data(), h->d_stack_s, h->n_int * sizeof(double), hipMemcpyDeviceToHost);
+            for (size_t j = 0; e == hipSuccess && j < h->n_int; ++j) illum[j] = (float)tmp[j];
+        }
+    } else {
+        if (e == hipSuccess && image)
+            e = hipMemcpy(image, h->d_image, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && illum)
+            e = hipMemcpy(illum, h->d_illum, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    h->times.d2h += t.elapsed();
+    return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image download");
+}
+
+int rtm_hip_image_get_f64(rtm_hip_engine* h, double* image, double* illum) {
+    if (int rc = check_handle(h, true)) return rc;
+    if (!h->image_stack) return h->fail(RTM_HIP_ERR_ARG, "engine was set up with an FP32 image stack");
+    (void)hipSetDevice(h->device);
+    rtm::Timer t;
+    hipError_t e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && image)
-        e = hipMemcpy(image, h->d_image, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+        e = hipMemcpy(image, h->d_stack_i, h->n_int * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && illum)
-        e = hipMemcpy(illum, h->d_illum, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+        e = hipMemcpy(illum, h->d_stack_s, h->n_int * sizeof(double), hipMemcpyDeviceToHost);
     h->times.d2h += t.elapsed();
     return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image download");
 }

@@ -9,7 +9,8 @@
 
 namespace rtm {
 
-HipRTM::HipRTM(int device, int snapshot_format) : snapshot_format_(snapshot_format) {
+HipRTM::HipRTM(int device, int snapshot_format, int image_stack)
+    : snapshot_format_(snapshot_format), image_stack_(image_stack) {
     rtm_hip_engine* h = nullptr;
     const int rc = rtm_hip_create(device, &h);
     if (rc != RTM_HIP_OK)
@@ -58,7 +59,7 @@ void HipRTM::setup(const VelocityModel& model, const RTMParams& par, const TimeA
         throw std::runtime_error("velocity model size does not match nx*nz");
     rtm_hip_grid g{model.grid.nx, model.grid.nz, model.grid.dx, model.grid.dz, model.grid.ox, model.grid.oz};
     rtm_hip_params p{par.nb, par.order, par.sponge_alpha, par.store_interval, par.f0, par.verbose ? 1 : 0,
-                     snapshot_format_};
+                     snapshot_format_, image_stack_};
     check(rtm_hip_setup(h_, &g, model.v.data(), &p, ta.nt, ta.dt));
     rtm_hip_info info;
     check(rtm_hip_get_info(h_, &info));

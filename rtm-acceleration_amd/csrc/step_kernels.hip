@@ -24,19 +24,26 @@
 // inject-then-damp.  Duplicate receiver cells add in ascending receiver order
 // through a per-tile entry table, which is the reference's sequential order.
 //
-// Two mappings (Family in step_kernels.hpp); both keep the arithmetic above.
+// 16-bit snapshots (no reference counterpart).  On a stored step the kernel
+// reduces max|p^{n+1}| over the interior (one atomic max per wave on the
+// float bits, all values >= 0); the next step's kernel, which reads that field
+// anyway as p^n, writes code = round(p^n * 2^k) to the 16-bit store, with
+// 2^k the power of two that keeps |code| < 2^15.  Scaling by 2^k is exact, so
+// the only error is the FP16/BF16 rounding; decoding is code * 2^-k, exact.
 //
-// kMarch (default).  Each wave64 owns one tile: a depth strip of 64
-// consecutive cells, of which the middle 64 - 2H lanes are outputs (strips
+// Two mappings (Family in step_kernels.hpp) share the arithmetic above.
+//
+// kMarch / kMarchDpp (default).  Each wave64 owns one tile: a depth strip of
+// 64 consecutive cells, of which the middle 64 - 2H lanes are outputs (strips
 // overlap by 2H cells), times tile_x columns.  A lane loads its depth row of
-// p^n over the tile_x + 2H columns the tile needs, and p^{n-1} and vdt2 over
-// the tile_x output columns, all before any arithmetic: tile_x*3 + 2H
+// p^n over the tile_x + 2H columns the tile needs, and every other operand of
+// its tile_x output columns, before any arithmetic: up to 3*tile_x + 2H
 // independent 256-byte row loads in flight per wave, no LDS, no barrier.
 // x-neighbours are in the lane's own registers; z-neighbours come from the
-// lanes k above/below through cross-lane permutes.  The step is bound by how
-// many bytes are in flight, not by instruction issue, so this layout trades a
-// 2H/64 depth overlap and a 2H/tile_x column overlap (both re-read from L2)
-// for one memory round trip per wave.
+// lanes k above/below through ds_bpermute or chained DPP lane shifts.  The
+// step is bound by the bytes in flight, not by instruction issue, so this
+// layout trades a 2H/64 depth overlap and a 2H/tile_x column overlap (both
+// re-read from L2) for one memory round trip per wave.
 //
 // kLdsTile.  A 256-thread workgroup (4 x wave64) owns 64 depth cells x tile_x
 // columns; the tile of p^n plus its halo is staged in LDS with 256-byte row
@@ -60,6 +67,85 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
     return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
 }
 
+// Value held by lane+1 / lane-1 of the wave (DPP wave_shl:1 / wave_shr:1, a
+// VALU move instead of an LDS-crossbar permute).  Lanes with no source read 0;
+// they are never output lanes for shifts up to H.
+__device__ __forceinline__ float from_lane_above(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_lane_below(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_max(float m) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, kTileZ));
+    return m;
+}
+
+// v * 2^k rounded to FP16 or BF16 (round to nearest even).
+__device__ __forceinline__ uint16_t encode_snapshot(float v, int k, int fmt) {
+    const float s = ldexpf(v, k);
+    if (fmt == (int)SnapFmt::BF16) {
+        unsigned b = __float_as_uint(s);
+        b += 0x7fffu + ((b >> 16) & 1u);
+        return (uint16_t)(b >> 16);
+    }
+    const _Float16 h = (_Float16)s;
+    uint16_t bits;
+    __builtin_memcpy(&bits, &h, sizeof bits);
+    return bits;
+}
+
+// Source / receiver injection and trace recording of one output cell, in
+// entry order (the tile's entries are sorted by cell, then code).
+template <Dir D>
+__device__ __forceinline__ float apply_entries(const StepArgs& a, int cell, float out, float v2, int e0,
+                                               int e1) {
+    for (int e = e0; e < e1; ++e) {
+        const Entry en = a.entries[e];
+        if (en.cell != cell) continue;
+        if constexpr (D == Dir::Forward) {
+            if (en.code < 0) out = out + v2 * a.wavelet[a.it];
+            else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
+        } else {
+            out = out + v2 * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
+        }
+    }
+    return out;
+}
+
+// What the snapshot/imaging epilogue does on this launch.
+struct SnapCtx {
+    bool imaging;      // backward stored step
+    bool store32;      // forward stored step, FP32 snapshot
+    bool store_max;    // forward stored step, 16-bit snapshot (max reduction)
+    bool quantise;     // forward, previous step was stored (16-bit)
+    int k_in;          // backward decode exponent
+    int k_q;           // forward encode exponent
+};
+
+__device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
+    SnapCtx c;
+    const bool wide = a.snap_fmt == (int)SnapFmt::F32;
+    c.imaging = d == Dir::Backward && (wide ? a.snap_in != nullptr : a.snap16_in != nullptr);
+    c.store32 = d == Dir::Forward && wide && a.snap_out != nullptr;
+    c.store_max = d == Dir::Forward && !wide && a.snap_max != nullptr;
+    c.quantise = d == Dir::Forward && !wide && a.quant_out != nullptr;
+    c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*a.snap16_max) : 0;
+    c.k_q = c.quantise ? snapshot_scale_exponent(*a.quant_max) : 0;
+    return c;
+}
+
+// Snapshot value f at interior index j for imaging (FP32 or decoded 16-bit).
+__device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx& c, ptrdiff_t j) {
+    if (a.snap_fmt == (int)SnapFmt::F32) return a.snap_in[j];
+    return decode_snapshot(a.snap16_in[j], c.k_in, (SnapFmt)a.snap_fmt);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tile mapping: 256 threads, 64 depth cells x TX = 4*RX columns.
+// ---------------------------------------------------------------------------
 template <int H, int RX, Dir D>
 __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     constexpr int TX = RX * kWaves;          // tile columns
@@ -72,13 +158,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     const int z0 = tz * kTileZ, x0 = tx * TX;
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int wave = (int)threadIdx.x / kTileZ;
+    const SnapCtx sc = snap_ctx(a, D);
 
     const int iz = z0 + lane;
     const bool z_ok = iz >= H && iz < a.nze - H;
+    const int jz = iz - a.nb;
+    const bool z_int = z_ok && jz >= 0 && jz < a.nz;
     const int lx0 = wave * RX;               // first local column of this wave
 
     // ---- the streamed operands first, so their latency overlaps the staging
-    float pv[RX], vv[RX], wxv[RX];
+    float pv[RX], vv[RX], wxv[RX], fv[RX], iv[RX], sv[RX];
 #pragma unroll
     for (int r = 0; r < RX; ++r) {
         const int ix = x0 + lx0 + r;
@@ -87,6 +176,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         pv[r] = ok ? a.p_prev[i] : 0.0f;
         vv[r] = ok ? a.vdt2[i] : 0.0f;
         wxv[r] = ok ? a.wx[ix] : 0.0f;
+        const int jx = ix - a.nb;
+        const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
+        const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+        fv[r] = in ? snapshot_value(a, sc, j) : 0.0f;
+        iv[r] = in ? a.image[j] : 0.0f;
+        sv[r] = in ? a.illum[j] : 0.0f;
     }
 
     // ---- stage p^n (tile + halo) in LDS; out-of-grid cells read as 0 -------
@@ -111,6 +206,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 
     const float wzv = a.wz[iz];
     const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+    float vmax = 0.0f;
 
 #pragma unroll
     for (int r = 0; r < RX; ++r) {
@@ -129,53 +225,38 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 
         const ptrdiff_t i = (ptrdiff_t)ix * a.ldz + iz;
         const float w = wxv[r] * wzv;
-        const float v2 = vv[r];
         const float prev = pv[r] * w;
-        float out = (2.0f * pc - prev) + v2 * lap;
+        float out = (2.0f * pc - prev) + vv[r] * lap;
         out = out * w;
-
-        // injection / recording entries of this tile (usually none)
-        for (int e = e0; e < e1; ++e) {
-            const Entry en = a.entries[e];
-            if (en.cell != (int)i) continue;
-            if constexpr (D == Dir::Forward) {
-                if (en.code < 0) out = out + v2 * a.wavelet[a.it];
-                else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
-            } else {
-                out = out + v2 * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
-            }
-        }
+        out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
         a.p_next[i] = out;
 
-        const int jx = ix - a.nb, jz = iz - a.nb;
-        if (jx >= 0 && jx < a.nx && jz >= 0 && jz < a.nz) {
+        const int jx = ix - a.nb;
+        if (z_int && jx >= 0 && jx < a.nx) {
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
             if constexpr (D == Dir::Forward) {
-                if (a.snap_out) a.snap_out[j] = out;
+                if (sc.store32) a.snap_out[j] = out;
+                if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
+                if (sc.quantise) a.quant_out[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
             } else {
-                if (a.snap_in) {
-                    const float f = a.snap_in[j];
-                    a.image[j] = a.image[j] + f * out;
-                    a.illum[j] = a.illum[j] + f * f;
+                if (sc.imaging) {
+                    a.image[j] = iv[r] + fv[r] * out;
+                    a.illum[j] = sv[r] + fv[r] * fv[r];
                 }
             }
         }
     }
+    if (sc.store_max) {
+        vmax = wave_max(vmax);
+        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
+    }
 }
 
-// Value held by lane+1 / lane-1 of the wave (DPP wave_shl:1 / wave_shr:1, a
-// VALU move instead of an LDS-crossbar permute).  Lanes with no source read 0;
-// they are never output lanes for shifts up to H.
-__device__ __forceinline__ float from_lane_above(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float from_lane_below(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-
-// Fused step, march mapping: one wave64 = one tile of (64 - 2H) x TX outputs.
-// DPP selects how depth neighbours cross lanes: chained one-lane DPP shifts,
-// or ds_bpermute (__shfl_up/__shfl_down) by k lanes.
+// ---------------------------------------------------------------------------
+// March mapping: one wave64 = one tile of (64 - 2H) x TX outputs.  DPP picks
+// how depth neighbours cross lanes: chained one-lane DPP shifts, or
+// ds_bpermute (__shfl_up/__shfl_down) by k lanes.
+// ---------------------------------------------------------------------------
 template <int H, int TX, Dir D, bool DPP>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
@@ -191,6 +272,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     const int x0 = H + tx * TX;              // first output column
     const bool z_load = z < a.nze;
     const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
+    const int jz = z - a.nb;                 // interior depth index
+    const bool z_int = z_out && jz >= 0 && jz < a.nz;
+    const SnapCtx sc = snap_ctx(a, D);
 
     // ---- every load of the tile, issued before any use ------------------
     float q[NQ];
@@ -199,13 +283,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         const int x = x0 - H + j;
         q[j] = (z_load && x < a.nxe) ? a.p_cur[(ptrdiff_t)x * a.ldz + z] : 0.0f;
     }
-    const int jz = z - a.nb;                 // interior depth index
-    const bool z_int = z_out && jz >= 0 && jz < a.nz;
     float pv[TX], vv[TX], wxv[TX];
-    // backward stored steps: snapshot, image and illumination of the tile
-    constexpr int NI = D == Dir::Backward ? TX : 1;
+    constexpr int NI = D == Dir::Backward ? TX : 1;   // imaging operands
     float fv[NI], iv[NI], sv[NI];
-    const bool imaging = D == Dir::Backward && a.snap_in != nullptr;
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
         const int x = x0 + r;
@@ -216,15 +296,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         wxv[r] = x < a.nxe ? a.wx[x] : 0.0f;
         if constexpr (D == Dir::Backward) {
             const int jx = x - a.nb;
-            const bool in = imaging && ok && z_int && jx >= 0 && jx < a.nx;
+            const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-            fv[r] = in ? a.snap_in[j] : 0.0f;
+            fv[r] = in ? snapshot_value(a, sc, j) : 0.0f;
             iv[r] = in ? a.image[j] : 0.0f;
             sv[r] = in ? a.illum[j] : 0.0f;
         }
     }
     const float wzv = z_load ? a.wz[z] : 0.0f;
     const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+    float vmax = 0.0f;
 
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
@@ -252,32 +333,49 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
         const float prev = pv[r] * w;
         float out = (2.0f * pc - prev) + vv[r] * lap;
         out = out * w;
-
-        for (int e = e0; e < e1; ++e) {
-            const Entry en = a.entries[e];
-            if (en.cell != (int)i) continue;
-            if constexpr (D == Dir::Forward) {
-                if (en.code < 0) out = out + vv[r] * a.wavelet[a.it];
-                else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
-            } else {
-                out = out + vv[r] * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
-            }
-        }
+        out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
         a.p_next[i] = out;
 
         const int jx = x - a.nb;
         if (z_int && jx >= 0 && jx < a.nx) {
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
             if constexpr (D == Dir::Forward) {
-                if (a.snap_out) a.snap_out[j] = out;
+                if (sc.store32) a.snap_out[j] = out;
+                if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
+                if (sc.quantise) a.quant_out[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
             } else {
-                if (imaging) {
+                if (sc.imaging) {
                     a.image[j] = iv[r] + fv[r] * out;
                     a.illum[j] = sv[r] + fv[r] * fv[r];
                 }
             }
         }
     }
+    if (sc.store_max) {
+        vmax = wave_max(vmax);
+        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
+    }
+}
+
+// Quantise the interior of an extended field into a 16-bit snapshot.
+__global__ __launch_bounds__(256) void k_quantise(const float* p, int ldz, int nb, int nx, int nz, int fmt,
+                                                  const float* maxv, uint16_t* out) {
+    const size_t n = (size_t)nx * nz;
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int jx = (int)(j / (size_t)nz), jz = (int)(j % (size_t)nz);
+    const int k = snapshot_scale_exponent(*maxv);
+    out[j] = encode_snapshot(p[(ptrdiff_t)(jx + nb) * ldz + jz + nb], k, fmt);
+}
+
+// FP64 cross-shot stack: stack += (double)image; image = 0.
+__global__ __launch_bounds__(256) void k_fold(float* image, float* illum, double* si, double* ss, size_t n) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    si[j] += (double)image[j];
+    ss[j] += (double)illum[j];
+    image[j] = 0.0f;
+    illum[j] = 0.0f;
 }
 
 // Launches a kernel; with events, through hipExtLaunchKernelGGL so that the
@@ -363,6 +461,21 @@ hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs&
                        hipEvent_t start, hipEvent_t stop) {
     return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, g, a, stream, start, stop)
                                : launch_dir<Dir::Backward>(half, g, a, stream, start, stop);
+}
+
+hipError_t launch_quantise(const float* p, int ldz, int nb, int nx, int nz, SnapFmt fmt, const float* max,
+                           uint16_t* out, hipStream_t stream) {
+    const size_t n = (size_t)nx * nz;
+    hipLaunchKernelGGL(k_quantise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, ldz, nb, nx, nz,
+                       (int)fmt, max, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_fp64(float* image, float* illum, double* stack_i, double* stack_s, size_t n,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(k_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, image, illum, stack_i,
+                       stack_s, n);
+    return hipGetLastError();
 }
 
 }  // namespace rtm_hip

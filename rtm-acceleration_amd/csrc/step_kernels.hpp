@@ -2,13 +2,16 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 #include <cstddef>
+#include <cstdint>
 
 namespace rtm_hip {
 
 constexpr int kMaxHalf = 6;
 constexpr int kTileZ = 64;        // one wave64 spans 64 consecutive depth cells
-constexpr int kWaves = 4;         // waves per workgroup, each owns tile_x/4 columns
+constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kBlockThreads = kTileZ * kWaves;
 
 // Stencil coefficients pre-divided by the spacing (src/rtm/rtm_cpu.cpp:75-79).
@@ -20,7 +23,7 @@ struct Coeffs {
 
 // An injection/recording entry of the per-tile table: the extended-grid cell
 // (device index ix*ldz+iz) and a code: -1 = add vdt2*wavelet[it] (source),
-// >= 0 = receiver ir (forward: record the cell into rec_out[ir*nt+it];
+// >= 0 = receiver ir (forward: record the cell into rec_out[ir*rec_stride+it];
 // backward: add vdt2*traces[ir*trace_stride+it]).
 struct Entry {
     int cell;
@@ -28,6 +31,11 @@ struct Entry {
 };
 
 enum class Dir : int { Forward = 0, Backward = 1 };
+
+// Snapshot storage: FP32 (the reference's), or 16-bit with one FP32 value per
+// snapshot, max|p| over the interior, from which an exact power-of-two scale
+// 2^k is derived (snapshot_scale_exponent).
+enum class SnapFmt : int { F32 = 0, F16 = 1, BF16 = 2 };
 
 struct StepArgs {
     // fields: p_prev holds p^{n-1} damped once; p_cur p^n; p_next p^{n+1}
@@ -39,9 +47,9 @@ struct StepArgs {
     const float* wz;          // [ldz]  Cerjan taper along z (0 in the pitch pad)
     Coeffs       coef;
     int nxe, nze, ldz, nb, nx, nz;
-    int ntz, ntx, ntiles;     // tile grid: ntz = ldz/64 rows of tiles along z
+    int ntz, ntx, ntiles;     // tile grid of the launch (StepGeometry)
     int it;
-    // per-tile injection / recording table (CSR over logical tiles)
+    // per-tile injection / recording table (CSR over tiles)
     const int*   tile_ptr;    // [ntiles+1]
     const Entry* entries;
     const float* wavelet;     // [nt]   forward source amplitude
@@ -49,10 +57,20 @@ struct StepArgs {
     long long    trace_stride;
     float*       rec_out;     // forward modelling: trace-major, stride rec_stride
     long long    rec_stride;
-    // forward: store the new field's interior into snap_out (nx*nz), or NULL
+    int          snap_fmt;    // SnapFmt
+    // forward, stored step: FP32 -> the new field's interior into snap_out;
+    // F16/BF16 -> max|p^{n+1}| over the interior into *snap_max (float bits)
     float*       snap_out;
-    // backward: image the new field against snap_in (nx*nz), or NULL
-    const float* snap_in;
+    unsigned*    snap_max;
+    // forward, step after a stored step (F16/BF16): quantise p^n's interior
+    // into quant_out with the scale of *quant_max
+    uint16_t*    quant_out;
+    const float* quant_max;
+    // backward, stored step: image the new field against the snapshot
+    // (snap_in for FP32; snap16_in + *snap16_max for F16/BF16), or no imaging
+    const float*    snap_in;
+    const uint16_t* snap16_in;
+    const float*    snap16_max;
     float*       image;
     float*       illum;
 };
@@ -64,14 +82,14 @@ struct StepArgs {
 //             lanes are outputs (strips overlap by 2H), and tile_x columns;
 //             it loads every operand of its tile up front (no LDS, no
 //             barrier) and takes depth neighbours from other lanes through
-//             ds_bpermute (kMarch) or chained DPP row shifts (kMarchDpp).
+//             ds_bpermute (kMarch) or chained DPP lane shifts (kMarchDpp).
 enum class Family : int { kLdsTile = 0, kMarch = 1, kMarchDpp = 2 };
 
 // Tile grid of one launch.  Tiles are numbered x-major over (ntz x ntx):
 // tile = tx * ntz + tz; the injection table is indexed the same way.
 struct StepGeometry {
-    Family family = Family::kMarch;
-    int tile_x = 16;     // output columns per tile
+    Family family = Family::kMarchDpp;
+    int tile_x = 4;      // output columns per tile
     int tile_z = 64;     // output depth cells per tile
     int x_origin = 0;    // first column of tile 0
     int z_origin = 0;    // first depth cell of tile 0
@@ -89,5 +107,45 @@ inline int tile_of_cell(const StepGeometry& g, int ixe, int ize) {
 // With start/stop events, they record the kernel's own start and end.
 hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
                        hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+
+// Quantise the interior of field p (extended layout) into a 16-bit snapshot
+// with the scale of *max (for a stored step that has no following step).
+hipError_t launch_quantise(const float* p, int ldz, int nb, int nx, int nz, SnapFmt fmt, const float* max,
+                           uint16_t* out, hipStream_t stream);
+
+// image/illum (FP32, n values) added into the FP64 stacks, then reset to zero.
+hipError_t launch_fold_fp64(float* image, float* illum, double* stack_i, double* stack_s, size_t n,
+                            hipStream_t stream);
+
+// Snapshot scale exponent k: |v| * 2^k < 2^15 for every |v| <= maxabs (0 when
+// maxabs is 0), so 16-bit codes hold v * 2^k and decode exactly as code * 2^-k.
+__host__ __device__ inline int snapshot_scale_exponent(float maxabs) {
+    if (!(maxabs > 0.0f)) return 0;
+    int e = 0;
+    (void)frexpf(maxabs, &e);      // maxabs = m 2^e, m in [0.5, 1)
+    return 15 - e;
+}
+
+// Decoding of a 16-bit snapshot code to the stored float (exact).
+__host__ __device__ inline float decode_snapshot(uint16_t code, int k, SnapFmt fmt) {
+    float f;
+    if (fmt == SnapFmt::BF16) {
+        const uint32_t bits = (uint32_t)code << 16;
+        __builtin_memcpy(&f, &bits, 4);
+    } else {
+        // IEEE binary16 -> binary32
+        const uint32_t s = (uint32_t)(code >> 15) << 31;
+        const uint32_t e = (code >> 10) & 0x1f;
+        const uint32_t m = code & 0x3ff;
+        if (e == 0) {
+            f = ldexpf((float)m, -24);           // zero or subnormal: m * 2^-24
+            if (s) f = -f;
+        } else {
+            const uint32_t bits = s | ((e == 31 ? 255u : e - 15 + 127) << 23) | (m << 13);
+            __builtin_memcpy(&f, &bits, 4);
+        }
+    }
+    return ldexpf(f, -k);
+}
 
 }  // namespace rtm_hip

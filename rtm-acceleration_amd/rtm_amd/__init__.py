@@ -13,11 +13,12 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import SNAP_BF16, SNAP_F16, SNAP_F32, STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP
+from ._capi import (SNAP_BF16, SNAP_F16, SNAP_F32, STACK_F32, STACK_F64, STEP_LDS_TILE, STEP_MARCH,
+                    STEP_MARCH_DPP)
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
 __all__ = ["HipEngine", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
-           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave", "bandwidth_probe",
+           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STACK_F32", "STACK_F64", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave", "bandwidth_probe",
            "library_path"]
 
 
@@ -84,10 +85,10 @@ class HipEngine:
 
     # ---- RTMEngine::setup -------------------------------------------------
     def setup(self, model: VelocityModel, params: RTMParams, nt: int, dt: float,
-              snapshot_format: int = SNAP_F32):
+              snapshot_format: int = SNAP_F32, image_stack: int = STACK_F32):
         g = _capi.Grid(model.nx, model.nz, model.dx, model.dz, model.ox, model.oz)
         p = _capi.Params(params.nb, params.order, params.sponge_alpha, params.store_interval,
-                         params.f0, int(params.verbose), snapshot_format)
+                         params.f0, int(params.verbose), snapshot_format, image_stack)
         v = np.ascontiguousarray(model.v, dtype=np.float32).reshape(-1)
         self._check(self.lib.rtm_hip_setup(self.h, C.byref(g), _capi.fptr(v), C.byref(p), nt, dt))
         self.model, self.params, self.nt, self.dt = model, params, nt, dt
@@ -135,6 +136,15 @@ class HipEngine:
         ill = np.empty(n, np.float32)
         self._check(self.lib.rtm_hip_image_get(self.h, _capi.fptr(img), _capi.fptr(ill)))
         return img, ill
+
+    def image_f64(self):
+        """FP64 stacks (engines set up with image_stack=STACK_F64)."""
+        n = self.n_int
+        img = np.empty(max(n, 1), np.float64)
+        ill = np.empty(max(n, 1), np.float64)
+        dp = C.POINTER(C.c_double)
+        self._check(self.lib.rtm_hip_image_get_f64(self.h, img.ctypes.data_as(dp), ill.ctypes.data_as(dp)))
+        return img[:n], ill[:n]
 
     def migrate(self, shots):
         keep: list = []

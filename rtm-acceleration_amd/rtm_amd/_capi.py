@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "librtm_hip.so")
 OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
 STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP = 0, 1, 2
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
+STACK_F32, STACK_F64 = 0, 1
 
 
 class Grid(C.Structure):
@@ -27,7 +28,7 @@ class Grid(C.Structure):
 class Params(C.Structure):
     _fields_ = [("nb", C.c_int), ("order", C.c_int), ("sponge_alpha", C.c_float),
                 ("store_interval", C.c_int), ("f0", C.c_float), ("verbose", C.c_int),
-                ("snapshot_format", C.c_int)]
+                ("snapshot_format", C.c_int), ("image_stack", C.c_int)]
 
 
 class Shot(C.Structure):
@@ -46,7 +47,7 @@ class Info(C.Structure):
                 ("nxe", C.c_int), ("nze", C.c_int), ("ldz", C.c_int), ("nt", C.c_int),
                 ("store_interval", C.c_int), ("num_snapshots", C.c_int),
                 ("snapshot_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-                ("snapshot_format", C.c_int), ("step_family", C.c_int * 2), ("tile_x", C.c_int * 2),
+                ("snapshot_format", C.c_int), ("image_stack", C.c_int), ("step_family", C.c_int * 2), ("tile_x", C.c_int * 2),
                 ("autotuned", C.c_int * 2),
                 ("device_name", C.c_char * 128)]
 
@@ -70,6 +71,7 @@ SIGNATURES = {
     "rtm_hip_backward": (C.c_int, [_H, C.POINTER(Shot)]),
     "rtm_hip_image_zero": (C.c_int, [_H]),
     "rtm_hip_image_get": (C.c_int, [_H, _f32p, _f32p]),
+    "rtm_hip_image_get_f64": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "rtm_hip_migrate": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _f32p, _f32p]),
     "rtm_hip_model_shot": (C.c_int, [_H, C.POINTER(Shot), _f32p]),
     "rtm_hip_get_times": (C.c_int, [_H, C.POINTER(Times)]),
