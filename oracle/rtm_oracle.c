@@ -242,7 +242,7 @@ int orc_map_geometry(orc_engine* e, const orc_shot* s, long long* src_out, long 
 #define ONE 1.0
 #define TWO 2.0
 #define PI_SQ (M_PI * M_PI)
-#define T0_NUM 1.2
+#define T0_NUM ((double)1.2f)
 #define EXPR exp
 #define CF(name) (e->name##_d)
 /* cpu-opt exists in FP32 only (orc_create refuses variant 1 here). */

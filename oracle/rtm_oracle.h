@@ -8,9 +8,10 @@
  * It restates, function by function, the reference's CPU engines:
  *   variant 0  "cpu"     CPUReferenceRTM   src/rtm/{rtm,rtm_cpu,forward,backward,imaging}.cpp
  *   variant 1  "cpu-opt" CPUOptimizedRTM   src/cpu_opt/cpu_opt.cpp:147-312
- * in FP32 (bit-identical to the reference's -O2 build) and in FP64 (the same
- * algorithm with every operation in double, used to state the rounding error
- * of an FP32 image).
+ * in FP32 (bit-identical to the reference's -O2 build) and in FP64 (an FP64
+ * build of the same code: every variable and operation in double, the
+ * reference's float literals -- the stencil coefficients, 1.2f -- keeping
+ * their float values; it states the rounding error of an FP32 image).
  *
  * Pinning (what checks each function; see tests/test_oracle_*.py):
  *   coefficients / abs-sum    exact fractions, sum == 0, polynomial exactness,

@@ -11,8 +11,12 @@ counterpart; the reference stores FP32 snapshots and stacks in FP32).
   and bounded at 1e-2.
 * The FP64 stack of one shot equals the FP32 image; over several shots it
   agrees with the FP32 running sum to rounding.
-* The 601 x 401 crop of the salt model at order 12: FP32 snapshots give
-  rel. L2 <= 1e-5 against the FP64 oracle, FP16 snapshots <= 1e-3.
+* The 601 x 401 crop of the salt model at order 12 (nt 10000): the GPU's FP32
+  image is bit-identical to the FP32 oracle (the reference's arithmetic), whose
+  own distance to the FP64 oracle is 1.6e-4 here: FP32 leapfrog stepping at
+  ~200 steps per period cancels about three of FP32's seven digits in
+  2p^n - p^{n-1} every step, a phase drift that grows linearly in time.  FP16
+  snapshots add < 1e-4 on top of that and stay <= 1e-3 from FP64.
 """
 import numpy as np
 import pytest
@@ -129,8 +133,12 @@ def test_salt_crop_against_fp64_oracle(hip_engine_cls):
         I32, _ = eng.migrate([shot])
         eng.setup(crop, params, nt, dt, snapshot_format=rtm_amd.SNAP_F16, image_stack=rtm_amd.STACK_F64)
         I16, _ = eng.migrate([shot])
+    Iref, _ = O.Oracle(crop, params, nt, dt, variant="cpu-opt").migrate([shot])
     I64, _ = O.Oracle(crop, params, nt, dt, precision=64).migrate([shot])
-    e32, e16 = rel_l2(I32, I64), rel_l2(I16, I64)
-    print(f"  rel L2 vs FP64 oracle: FP32 snapshots {e32:.3e}, FP16 snapshots {e16:.3e}")
-    assert e32 <= 1e-5
+    e32, e16, q16 = rel_l2(I32, I64), rel_l2(I16, I64), rel_l2(I16, I32)
+    print(f"  rel L2 vs FP64 oracle: FP32 snapshots {e32:.3e}, FP16 snapshots {e16:.3e}; "
+          f"FP16 vs FP32 snapshots {q16:.3e}")
+    assert bits_equal(I32, Iref)          # the reference's FP32 arithmetic, exactly
+    assert e32 <= 3e-4                    # that arithmetic's own distance to FP64
+    assert q16 <= 1e-4                    # what 16-bit snapshots add
     assert e16 <= 1e-3
