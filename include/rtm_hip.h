@@ -204,6 +204,94 @@ int         rtm_hip_bandwidth_probe(int device, size_t bytes, int repeats, doubl
 void        rtm_hip_mute_direct_wave(const rtm_hip_shot* shot, float* traces,
                                      float v_surface, float t_pad, float t_taper);
 
+/* ------------------------------------------------------------------------
+ * Batched and served migration (rtm_hip_batch_*).
+ *
+ * The reference's engine binds one model, one RTMParams and one time axis per
+ * instance (include/rtm_engine.hpp:25-27) and migrates shots one after the
+ * other (src/rtm/rtm.cpp:83-103).  A batch handle instead takes independent
+ * shot-migration jobs, each with its own grid, velocity, order, store
+ * interval and nt, and advances every job on the device by one time step per
+ * tick, all jobs of one stencil order in one launch (a tile table spans the
+ * jobs), forward and backward phases mixed.  Jobs are admitted from a FIFO
+ * queue when their device footprint fits the memory budget; finished jobs
+ * free their memory and queued jobs take it (continuous batching).  A job's
+ * image is bit-identical to its image from rtm_hip_migrate of that one shot
+ * on a plain handle with the same snapshot format.  No reference counterpart.
+ * ------------------------------------------------------------------------ */
+#define RTM_HIP_JOB_QUEUED   0
+#define RTM_HIP_JOB_RUNNING  1
+#define RTM_HIP_JOB_DONE     2
+#define RTM_HIP_JOB_FAILED   3
+
+typedef struct {
+    size_t memory_budget;        /* device bytes for jobs; 0 = 90% of free memory    */
+    int    max_active;           /* cap on jobs on the device at once; 0 = no cap    */
+    int    use_graph;            /* 1: capture one tick in a hipGraph and replay it  */
+} rtm_hip_batch_config;
+
+/* One shot migration: the arguments of setup (grid, velocity, params, nt, dt)
+ * and of migrate for one shot.  params->snapshot_format applies per job;
+ * params->image_stack is ignored (see stack_ix0).  shot->traces is required,
+ * with shot->nt >= nt samples per trace.  stack_ix0/stack_iz0 >= 0 add the
+ * finished job's image and illumination into the handle's FP64 stack at
+ * that offset (rtm_hip_batch_set_stack); -1 leaves the stack alone. */
+typedef struct {
+    rtm_hip_grid   grid;
+    const float*   v;            /* nx*nz interior velocity */
+    rtm_hip_params params;
+    int            nt;
+    float          dt;
+    rtm_hip_shot   shot;
+    int            stack_ix0, stack_iz0;
+} rtm_hip_job;
+
+typedef struct {
+    long long jobs_submitted, jobs_done;
+    int       jobs_active, jobs_queued;
+    long long ticks;             /* time steps run (each advances every active job) */
+    long long rebuilds;          /* launch-table rebuilds (job set changes)         */
+    double    cell_steps;        /* 2*nt*nxe*nze summed over finished jobs          */
+    double    run_seconds;       /* wall time inside rtm_hip_batch_run             */
+    double    busy_latency_s;    /* sum of submit-to-done times of finished jobs   */
+    size_t    bytes_in_use, bytes_budget;
+    int       ngroups;           /* stencil half-widths with a launch per tick     */
+    int       group_half[6], group_tile_x[6], group_dpp[6];
+} rtm_hip_batch_stats;
+
+typedef struct rtm_hip_batch rtm_hip_batch;
+
+/* Creates a batch handle on `device` (cfg NULL: defaults). */
+int         rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rtm_hip_batch** out);
+void        rtm_hip_batch_destroy(rtm_hip_batch* b);
+const char* rtm_hip_batch_last_error(const rtm_hip_batch* b);
+
+/* Device bytes the job would hold while active (admission figure). */
+int         rtm_hip_batch_job_bytes(rtm_hip_batch* b, const rtm_hip_job* job, size_t* bytes);
+
+/* Validates and queues a job (inputs copied; the caller's buffers are free on
+ * return) and returns its id.  Nothing runs until rtm_hip_batch_run. */
+int         rtm_hip_batch_submit(rtm_hip_batch* b, const rtm_hip_job* job, long long* id);
+
+/* Admits queued jobs and runs ticks.  until_done = 1: until no job is queued
+ * or active (or max_ticks > 0 ticks have run).  until_done = 0: at most
+ * max_ticks ticks (max_ticks <= 0: until the next job finishes).  Jobs that
+ * finish are retired (image downloaded, stack updated, memory freed). */
+int         rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done);
+
+/* State (RTM_HIP_JOB_*) and submit-to-done latency (-1 until done). */
+int         rtm_hip_batch_job_state(rtm_hip_batch* b, long long id, int* state, double* latency_s);
+
+/* Image and illumination (nx*nz floats each, either may be NULL) of a
+ * finished job; the job is forgotten afterwards. */
+int         rtm_hip_batch_result(rtm_hip_batch* b, long long id, float* image, float* illum);
+
+/* Allocates (and zeroes) an nx*nz FP64 image/illumination stack. */
+int         rtm_hip_batch_set_stack(rtm_hip_batch* b, int nx, int nz);
+int         rtm_hip_batch_get_stack(rtm_hip_batch* b, double* image, double* illum);
+
+int         rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,93 +55,13 @@
 
 #include <hip/hip_ext.h>
 
+#include "march_tile.hpp"
+
 namespace rtm_hip {
 
 namespace {
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  Give XCD k the
-// k-th contiguous block of logical tiles so neighbouring tiles share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-    const int q = ntiles >> 3, r = ntiles & 7;
-    const int xcd = b & 7, loc = b >> 3;
-    return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
-}
-
-// Value held by lane+1 / lane-1 of the wave (DPP wave_shl:1 / wave_shr:1, a
-// VALU move instead of an LDS-crossbar permute).  Lanes with no source read 0;
-// they are never output lanes for shifts up to H.
-__device__ __forceinline__ float from_lane_above(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float from_lane_below(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-
-__device__ __forceinline__ float wave_max(float m) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, kTileZ));
-    return m;
-}
-
-// v * 2^k rounded to FP16 or BF16 (round to nearest even).
-__device__ __forceinline__ uint16_t encode_snapshot(float v, int k, int fmt) {
-    const float s = ldexpf(v, k);
-    if (fmt == (int)SnapFmt::BF16) {
-        unsigned b = __float_as_uint(s);
-        b += 0x7fffu + ((b >> 16) & 1u);
-        return (uint16_t)(b >> 16);
-    }
-    const _Float16 h = (_Float16)s;
-    uint16_t bits;
-    __builtin_memcpy(&bits, &h, sizeof bits);
-    return bits;
-}
-
-// Source / receiver injection and trace recording of one output cell, in
-// entry order (the tile's entries are sorted by cell, then code).
-template <Dir D>
-__device__ __forceinline__ float apply_entries(const StepArgs& a, int cell, float out, float v2, int e0,
-                                               int e1) {
-    for (int e = e0; e < e1; ++e) {
-        const Entry en = a.entries[e];
-        if (en.cell != cell) continue;
-        if constexpr (D == Dir::Forward) {
-            if (en.code < 0) out = out + v2 * a.wavelet[a.it];
-            else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
-        } else {
-            out = out + v2 * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
-        }
-    }
-    return out;
-}
-
-// What the snapshot/imaging epilogue does on this launch.
-struct SnapCtx {
-    bool imaging;      // backward stored step
-    bool store32;      // forward stored step, FP32 snapshot
-    bool store_max;    // forward stored step, 16-bit snapshot (max reduction)
-    bool quantise;     // forward, previous step was stored (16-bit)
-    int k_in;          // backward decode exponent
-    int k_q;           // forward encode exponent
-};
-
-__device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
-    SnapCtx c;
-    const bool wide = a.snap_fmt == (int)SnapFmt::F32;
-    c.imaging = d == Dir::Backward && (wide ? a.snap_in != nullptr : a.snap16_in != nullptr);
-    c.store32 = d == Dir::Forward && wide && a.snap_out != nullptr;
-    c.store_max = d == Dir::Forward && !wide && a.snap_max != nullptr;
-    c.quantise = d == Dir::Forward && !wide && a.quant_out != nullptr;
-    c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*a.snap16_max) : 0;
-    c.k_q = c.quantise ? snapshot_scale_exponent(*a.quant_max) : 0;
-    return c;
-}
-
-// Snapshot value f at interior index j for imaging (FP32 or decoded 16-bit).
-__device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx& c, ptrdiff_t j) {
-    if (a.snap_fmt == (int)SnapFmt::F32) return a.snap_in[j];
-    return decode_snapshot(a.snap16_in[j], c.k_in, (SnapFmt)a.snap_fmt);
-}
+using namespace device;
 
 // ---------------------------------------------------------------------------
 // LDS-tile mapping: 256 threads, 64 depth cells x TX = 4*RX columns.
@@ -173,13 +93,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         const int ix = x0 + lx0 + r;
         const bool ok = z_ok && ix >= H && ix < a.nxe - H;
         const ptrdiff_t i = (ptrdiff_t)(ok ? ix : 0) * a.ldz + iz;
-        pv[r] = ok ? a.p_prev[i] : 0.0f;
+        pv[r] = (ok && !a.zero_prev) ? a.p_prev[i] : 0.0f;
         vv[r] = ok ? a.vdt2[i] : 0.0f;
         wxv[r] = ok ? a.wx[ix] : 0.0f;
         const int jx = ix - a.nb;
         const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
         const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-        fv[r] = in ? snapshot_value(a, sc, j) : 0.0f;
+        fv[r] = in ? snapshot_value(a, sc, j, i) : 0.0f;
         iv[r] = in ? a.image[j] : 0.0f;
         sv[r] = in ? a.illum[j] : 0.0f;
     }
@@ -187,7 +107,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     // ---- stage p^n (tile + halo) in LDS; out-of-grid cells read as 0 -------
     for (int r = wave; r < LX; r += kWaves) {
         const int gx = x0 - H + r;
-        const bool xin = gx >= 0 && gx < a.nxe;
+        const bool xin = gx >= 0 && gx < a.nxe && !a.zero_cur;
         const float* row = a.p_cur + (ptrdiff_t)(xin ? gx : 0) * a.ldz;
         tile[r * LZ + H + lane] = xin ? row[z0 + lane] : 0.0f;
         if (lane < 2 * H) {
@@ -239,7 +159,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
                 if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
                 if (sc.quantise) a.quant_out[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
             } else {
-                if (sc.imaging) {
+                if (sc.imaging && !a.dry) {
                     a.image[j] = iv[r] + fv[r] * out;
                     a.illum[j] = sv[r] + fv[r] * fv[r];
                 }
@@ -252,109 +172,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// March mapping: one wave64 = one tile of (64 - 2H) x TX outputs.  DPP picks
-// how depth neighbours cross lanes: chained one-lane DPP shifts, or
-// ds_bpermute (__shfl_up/__shfl_down) by k lanes.
-// ---------------------------------------------------------------------------
+// March mapping: one wave per tile (march_tile.hpp).
 template <int H, int TX, Dir D, bool DPP>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
-    constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
-    constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
-
     const int wave = (int)threadIdx.x / kTileZ;
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (t >= a.ntiles) return;               // whole wave leaves together
-    const int tz = t % a.ntz, tx = t / a.ntz;
-    const int z = tz * TZ + lane;            // depth cell held by this lane
-    const int x0 = H + tx * TX;              // first output column
-    const bool z_load = z < a.nze;
-    const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
-    const int jz = z - a.nb;                 // interior depth index
-    const bool z_int = z_out && jz >= 0 && jz < a.nz;
-    const SnapCtx sc = snap_ctx(a, D);
-
-    // ---- every load of the tile, issued before any use ------------------
-    float q[NQ];
-#pragma unroll
-    for (int j = 0; j < NQ; ++j) {
-        const int x = x0 - H + j;
-        q[j] = (z_load && x < a.nxe) ? a.p_cur[(ptrdiff_t)x * a.ldz + z] : 0.0f;
-    }
-    float pv[TX], vv[TX], wxv[TX];
-    constexpr int NI = D == Dir::Backward ? TX : 1;   // imaging operands
-    float fv[NI], iv[NI], sv[NI];
-#pragma unroll
-    for (int r = 0; r < TX; ++r) {
-        const int x = x0 + r;
-        const bool ok = z_out && x < a.nxe - H;
-        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
-        pv[r] = ok ? a.p_prev[i] : 0.0f;
-        vv[r] = ok ? a.vdt2[i] : 0.0f;
-        wxv[r] = x < a.nxe ? a.wx[x] : 0.0f;
-        if constexpr (D == Dir::Backward) {
-            const int jx = x - a.nb;
-            const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
-            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-            fv[r] = in ? snapshot_value(a, sc, j) : 0.0f;
-            iv[r] = in ? a.image[j] : 0.0f;
-            sv[r] = in ? a.illum[j] : 0.0f;
-        }
-    }
-    const float wzv = z_load ? a.wz[z] : 0.0f;
-    const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
-    float vmax = 0.0f;
-
-#pragma unroll
-    for (int r = 0; r < TX; ++r) {
-        const float pc = q[r + H];
-        float lap = a.coef.c0 * pc;
-        float zp = pc, zm = pc;
-#pragma unroll
-        for (int k = 1; k <= H; ++k) {
-            if constexpr (DPP) {
-                zp = from_lane_above(zp);                           // p^n at z + k
-                zm = from_lane_below(zm);                           // p^n at z - k
-            } else {
-                zp = __shfl_down(pc, (unsigned)k, kTileZ);
-                zm = __shfl_up(pc, (unsigned)k, kTileZ);
-            }
-            const float tx_k = a.coef.cx[k] * (q[r + H + k] + q[r + H - k]);
-            const float tz_k = a.coef.cz[k] * (zp + zm);
-            lap = lap + (tx_k + tz_k);
-        }
-        const int x = x0 + r;
-        if (!z_out || x >= a.nxe - H) continue;
-
-        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
-        const float w = wxv[r] * wzv;
-        const float prev = pv[r] * w;
-        float out = (2.0f * pc - prev) + vv[r] * lap;
-        out = out * w;
-        out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
-        a.p_next[i] = out;
-
-        const int jx = x - a.nb;
-        if (z_int && jx >= 0 && jx < a.nx) {
-            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-            if constexpr (D == Dir::Forward) {
-                if (sc.store32) a.snap_out[j] = out;
-                if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-                if (sc.quantise) a.quant_out[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
-            } else {
-                if (sc.imaging) {
-                    a.image[j] = iv[r] + fv[r] * out;
-                    a.illum[j] = sv[r] + fv[r] * fv[r];
-                }
-            }
-        }
-    }
-    if (sc.store_max) {
-        vmax = wave_max(vmax);
-        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
-    }
+    march_tile<H, TX, D, DPP>(a, t, lane);
 }
 
 // Quantise the interior of an extended field into a 16-bit snapshot.

@@ -73,6 +73,15 @@ struct StepArgs {
     const float*    snap16_max;
     float*       image;
     float*       illum;
+    // batched phases: the first steps of a phase read p_prev/p_cur as zero
+    // instead of their buffers (no reset pass between phases); a 16-bit
+    // snapshot of the forward phase's last step is taken, at the backward
+    // phase's first step, straight from pending_field (its extended field);
+    // dry runs write only p_next: no image, illumination or snapshot update
+    // (timing candidates on live state)
+    int          zero_prev, zero_cur;
+    const float* pending_field;
+    int          dry;
 };
 
 // Kernel families of the fused step.

@@ -13,11 +13,11 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import (SNAP_BF16, SNAP_F16, SNAP_F32, STACK_F32, STACK_F64, STEP_LDS_TILE, STEP_MARCH,
-                    STEP_MARCH_DPP)
+from ._capi import (JOB_DONE, JOB_FAILED, JOB_QUEUED, JOB_RUNNING, SNAP_BF16, SNAP_F16, SNAP_F32,
+                    STACK_F32, STACK_F64, STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP)
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
-__all__ = ["HipEngine", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
+__all__ = ["HipEngine", "BatchEngine", "JOB_QUEUED", "JOB_RUNNING", "JOB_DONE", "JOB_FAILED", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
            "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STACK_F32", "STACK_F64", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave", "bandwidth_probe",
            "library_path"]
 
@@ -186,3 +186,109 @@ class HipEngine:
         """Step-kernel mapping (STEP_MARCH / STEP_LDS_TILE, tile width 4/8/16);
         -1 / 0 select the built-in choice.  Results do not depend on it."""
         self._check(self.lib.rtm_hip_set_step_kernel(self.h, family, tile_x))
+
+
+class BatchEngine:
+    """Batched / served migration (rtm_hip_batch_*): independent shot jobs,
+    each on its own grid, advanced together one time step per tick."""
+
+    def __init__(self, device: int = 0, memory_budget: int = 0, max_active: int = 0,
+                 use_graph: bool = True):
+        self.lib = _capi.load()
+        cfg = _capi.BatchConfig(int(memory_budget), int(max_active), int(bool(use_graph)))
+        h = C.c_void_p()
+        rc = self.lib.rtm_hip_batch_create(device, C.byref(cfg), C.byref(h))
+        if rc != _capi.OK:
+            raise RuntimeError(self.lib.rtm_hip_batch_last_error(None).decode())
+        self.h = h
+        self.sizes: dict[int, int] = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rtm_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != _capi.OK:
+            raise RuntimeError(self.lib.rtm_hip_batch_last_error(self.h).decode())
+
+    @staticmethod
+    def _job(model: VelocityModel, params: RTMParams, nt: int, dt: float, shot: ShotRecord,
+             snapshot_format: int, stack_at, keep: list) -> _capi.Job:
+        j = _capi.Job()
+        j.grid = _capi.Grid(model.nx, model.nz, model.dx, model.dz, model.ox, model.oz)
+        v = np.ascontiguousarray(model.v, dtype=np.float32).reshape(-1)
+        keep.append(v)
+        j.v = _capi.fptr(v)
+        j.params = _capi.Params(params.nb, params.order, params.sponge_alpha, params.store_interval,
+                                params.f0, int(params.verbose), snapshot_format, STACK_F32)
+        j.nt, j.dt = int(nt), float(dt)
+        j.shot = _capi.shot_struct(shot, keep)
+        j.stack_ix0, j.stack_iz0 = (-1, -1) if stack_at is None else (int(stack_at[0]), int(stack_at[1]))
+        return j
+
+    def job_bytes(self, model, params, nt, dt, shot, snapshot_format: int = SNAP_F32) -> int:
+        keep: list = []
+        j = self._job(model, params, nt, dt, shot, snapshot_format, None, keep)
+        out = C.c_size_t()
+        self._check(self.lib.rtm_hip_batch_job_bytes(self.h, C.byref(j), C.byref(out)))
+        return int(out.value)
+
+    def submit(self, model: VelocityModel, params: RTMParams, nt: int, dt: float, shot: ShotRecord,
+               snapshot_format: int = SNAP_F32, stack_at=None) -> int:
+        """Queue one shot migration; stack_at = (ix0, iz0) adds its image into
+        the FP64 stack at that offset when it finishes."""
+        keep: list = []
+        j = self._job(model, params, nt, dt, shot, snapshot_format, stack_at, keep)
+        jid = C.c_longlong()
+        self._check(self.lib.rtm_hip_batch_submit(self.h, C.byref(j), C.byref(jid)))
+        self.sizes[jid.value] = model.nx * model.nz
+        return int(jid.value)
+
+    def run(self, max_ticks: int = 0, until_done: bool = True):
+        self._check(self.lib.rtm_hip_batch_run(self.h, int(max_ticks), int(bool(until_done))))
+
+    def state(self, jid: int):
+        st, lat = C.c_int(), C.c_double()
+        self._check(self.lib.rtm_hip_batch_job_state(self.h, jid, C.byref(st), C.byref(lat)))
+        return int(st.value), float(lat.value)
+
+    def result(self, jid: int):
+        n = self.sizes[jid]
+        img = np.empty(n, np.float32)
+        ill = np.empty(n, np.float32)
+        self._check(self.lib.rtm_hip_batch_result(self.h, jid, _capi.fptr(img), _capi.fptr(ill)))
+        del self.sizes[jid]
+        return img, ill
+
+    def set_stack(self, nx: int, nz: int):
+        self._check(self.lib.rtm_hip_batch_set_stack(self.h, nx, nz))
+        self.stack_shape = (nx, nz)
+
+    def stack(self):
+        n = self.stack_shape[0] * self.stack_shape[1]
+        img = np.empty(n, np.float64)
+        ill = np.empty(n, np.float64)
+        dp = C.POINTER(C.c_double)
+        self._check(self.lib.rtm_hip_batch_get_stack(self.h, img.ctypes.data_as(dp), ill.ctypes.data_as(dp)))
+        return img, ill
+
+    def stats(self) -> dict:
+        s = _capi.BatchStats()
+        self._check(self.lib.rtm_hip_batch_get_stats(self.h, C.byref(s)))
+        out = {n: getattr(s, n) for n, _ in _capi.BatchStats._fields_}
+        k = s.ngroups
+        out["groups"] = [{"half": s.group_half[i], "tile_x": s.group_tile_x[i], "dpp": s.group_dpp[i]}
+                         for i in range(k)]
+        for n in ("group_half", "group_tile_x", "group_dpp"):
+            del out[n]
+        return out

@@ -18,6 +18,7 @@ OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
 STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP = 0, 1, 2
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 STACK_F32, STACK_F64 = 0, 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 
 class Grid(C.Structure):
@@ -57,6 +58,24 @@ class KernelStats(C.Structure):
                 ("sampled_ms", C.c_double * 2)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_size_t), ("max_active", C.c_int), ("use_graph", C.c_int)]
+
+
+class Job(C.Structure):
+    _fields_ = [("grid", Grid), ("v", C.POINTER(C.c_float)), ("params", Params), ("nt", C.c_int),
+                ("dt", C.c_float), ("shot", Shot), ("stack_ix0", C.c_int), ("stack_iz0", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_longlong), ("jobs_done", C.c_longlong),
+                ("jobs_active", C.c_int), ("jobs_queued", C.c_int), ("ticks", C.c_longlong),
+                ("rebuilds", C.c_longlong), ("cell_steps", C.c_double), ("run_seconds", C.c_double),
+                ("busy_latency_s", C.c_double), ("bytes_in_use", C.c_size_t),
+                ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
+                ("group_tile_x", C.c_int * 6), ("group_dpp", C.c_int * 6)]
+
+
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.
 _f32p = C.POINTER(C.c_float)
 _H = C.c_void_p
@@ -81,6 +100,17 @@ SIGNATURES = {
     "rtm_hip_set_step_kernel": (C.c_int, [_H, C.c_int, C.c_int]),
     "rtm_hip_mute_direct_wave": (None, [C.POINTER(Shot), _f32p, C.c_float, C.c_float, C.c_float]),
     "rtm_hip_bandwidth_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "rtm_hip_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
+    "rtm_hip_batch_destroy": (None, [_H]),
+    "rtm_hip_batch_last_error": (C.c_char_p, [_H]),
+    "rtm_hip_batch_job_bytes": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_size_t)]),
+    "rtm_hip_batch_submit": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_longlong)]),
+    "rtm_hip_batch_run": (C.c_int, [_H, C.c_int, C.c_int]),
+    "rtm_hip_batch_job_state": (C.c_int, [_H, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "rtm_hip_batch_result": (C.c_int, [_H, C.c_longlong, _f32p, _f32p]),
+    "rtm_hip_batch_set_stack": (C.c_int, [_H, C.c_int, C.c_int]),
+    "rtm_hip_batch_get_stack": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "rtm_hip_batch_get_stats": (C.c_int, [_H, C.POINTER(BatchStats)]),
 }
 
 _lib = None

@@ -14,6 +14,9 @@ sources).  Nothing here runs on the GPU host; the fixtures it writes are data:
   tiny_cases.npz       small grids (orders 2..12, store intervals, duplicate and
                        clamped receivers, truncated nt): modelled traces and
                        migrated images from the reference "cpu" engine
+  builtin/manifest.json  sha256 of the velocity and RTMS files the reference's
+                       rtm_synth writes for its built-in model with the
+                       scripts/run_synthetic.sh settings (4 shots, nt 1400)
   marmousi/            the reference's Marmousi velocity (data/real/), the
                        reference-held sha256 of the 12-shot RTMS file
                        (data/real/marmousi_dataset.sha256:1) and per-shot
@@ -97,6 +100,21 @@ def make_layered3() -> None:
                        "illummap.f32": sha256_file(os.path.join(tmp, "image_illummap.bin"))},
             "shots_bytes": os.path.getsize(shots),
         }
+    with open(os.path.join(out, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+
+
+def make_builtin() -> None:
+    out = os.path.join(HERE, "builtin")
+    os.makedirs(out, exist_ok=True)
+    args = ["--nx", "201", "--nz", "151", "--dx", "10", "--dz", "10", "--nt", "1400", "--dt", "0.001",
+            "--f0", "12", "--nshots", "4"]
+    with tempfile.TemporaryDirectory() as tmp:
+        run([os.path.join(REF_BIN, "rtm_synth"), "--outdir", tmp] + args)
+        manifest = {"what": "reference rtm_synth built-in model (scripts/run_synthetic.sh:14-16)",
+                    "args": args,
+                    "sha256": {"velocity.bin": sha256_file(os.path.join(tmp, "velocity.bin")),
+                               "shots.bin": sha256_file(os.path.join(tmp, "shots.bin"))}}
     with open(os.path.join(out, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
 
@@ -198,7 +216,11 @@ def make_marmousi() -> None:
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built; run make -C oracle where the reference sources exist")
+    if "--only-builtin" in sys.argv:
+        make_builtin()
+        sys.exit(0)
     make_layered3()
+    make_builtin()
     make_tiny()
     if "--skip-marmousi" not in sys.argv:
         make_marmousi()

@@ -53,3 +53,31 @@ def test_mute_matches_oracle():
         rtm_amd.mute_direct_wave(s, v, tp, tp)
         ref = O.mute_direct_wave(O.Shot(650.0, 20.0, rx, rz, nt, dt, tr), v, tp, tp)
         assert bits_equal(s.traces, ref)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes structure has the size and field offsets the C compiler
+    gives the header's struct (a mismatch would pass garbage through the ABI)."""
+    from rtm_amd import _capi
+    pairs = {"rtm_hip_grid": _capi.Grid, "rtm_hip_params": _capi.Params, "rtm_hip_shot": _capi.Shot,
+             "rtm_hip_times": _capi.Times, "rtm_hip_info": _capi.Info,
+             "rtm_hip_kernel_stats": _capi.KernelStats, "rtm_hip_batch_config": _capi.BatchConfig,
+             "rtm_hip_job": _capi.Job, "rtm_hip_batch_stats": _capi.BatchStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rtm_hip.h"', 'int main(void) {']
+    for cname, cls in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = []
+    for cname, cls in pairs.items():
+        want.append(f"{cname} size {C.sizeof(cls)}")
+        for f in cls._fields_:
+            want.append(f"{cname} {f[0]} {getattr(cls, f[0]).offset}")
+    assert [g for g in got if g] == want

@@ -1,0 +1,163 @@
+"""Batched engine: every job's image in a heterogeneous batch is bit-identical
+to the same shot migrated alone by the plain engine ("alone == in the mix").
+
+Jobs differ in grid, stencil order, nt, store interval, snapshot format and
+trace stride; max_active and the memory budget force jobs to wait and join
+while others are mid-way (forward and backward phases in one launch).  The
+plain engine's FP32 image is itself bit-identical to the oracle (and so to
+the reference's CPU path), which the first test re-checks for F32 jobs.
+Each case prints its parameters first so a failure names it.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal, line_shots, small_model
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+# (nx, nz, order, nb, nt, store, snapshot format, extra trace samples)
+JOBS = [
+    (61, 41, 8, 20, 180, 2, 0, 0),
+    (45, 33, 12, 14, 150, 1, 0, 7),
+    (80, 29, 4, 12, 210, 3, 1, 0),
+    (52, 47, 8, 16, 121, 4, 2, 0),
+    (37, 25, 2, 8, 97, 1, 1, 3),
+    (70, 38, 12, 18, 165, 5, 0, 0),
+    (33, 64, 6, 10, 133, 2, 2, 0),
+    (90, 31, 10, 16, 190, 7, 1, 0),
+]
+
+
+def _job(k, nx, nz, order, nb, nt, store, fmt, extra, seed=11):
+    import rtm_amd
+    model = small_model(nx, nz)
+    rng = np.random.default_rng(seed + k)
+    model.v = (model.v.reshape(nx, nz) * rng.uniform(0.95, 1.05, (nx, nz))).astype(np.float32).reshape(-1)
+    params = rtm_amd.RTMParams(nb=nb, order=order, store_interval=store, f0=15.0)
+    dt = 0.001
+    sx = float(rng.uniform(0, (nx - 1) * model.dx))
+    shot = line_shots(model, nt + extra, dt, [sx])[0]
+    shot.traces = (rng.standard_normal((shot.nrec, nt + extra)) * 1e-2).astype(np.float32)
+    return model, params, nt, dt, shot, fmt
+
+
+def _alone(hip_engine_cls, model, params, nt, dt, shot, fmt):
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, nt, dt, snapshot_format=fmt)
+        return eng.migrate([shot])
+
+
+@pytest.fixture(scope="module")
+def jobs():
+    return [_job(k, *j) for k, j in enumerate(JOBS)]
+
+
+@pytest.fixture(scope="module")
+def alone(hip_engine_cls, jobs):
+    return [_alone(hip_engine_cls, *j) for j in jobs]
+
+
+def test_alone_matches_oracle_for_f32(jobs, alone):
+    for k, (model, params, nt, dt, shot, fmt) in enumerate(jobs):
+        if fmt != 0:
+            continue
+        print(f"case: job {k} alone vs oracle", flush=True)
+        I_ref, S_ref = O.Oracle(model, params, nt, dt).migrate([shot])
+        assert bits_equal(I_ref, alone[k][0])
+        assert bits_equal(S_ref, alone[k][1])
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("max_active", [0, 3])
+def test_batch_equals_alone(jobs, alone, use_graph, max_active):
+    import rtm_amd
+    print(f"case: batch of {len(jobs)} jobs graph={use_graph} max_active={max_active}", flush=True)
+    with rtm_amd.BatchEngine(0, max_active=max_active, use_graph=use_graph) as b:
+        ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+        b.run()
+        st = b.stats()
+        assert st["jobs_done"] == len(jobs) and st["jobs_active"] == 0 and st["jobs_queued"] == 0
+        longest = max(2 * j[2] for j in jobs)
+        if max_active == 0:
+            assert st["ticks"] == longest          # every job ran from the first tick
+        else:
+            assert st["ticks"] > longest
+        if max_active:
+            assert st["rebuilds"] > 2      # jobs joined mid-run
+        for k, jid in enumerate(ids):
+            state, lat = b.state(jid)
+            assert state == rtm_amd.JOB_DONE and lat > 0
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}: image max diff {np.abs(alone[k][0] - I).max()}"
+            assert bits_equal(alone[k][1], S), f"job {k}: illumination differs"
+
+
+def test_interleaved_submission_and_budget(jobs, alone):
+    """Jobs submitted while others run, under a budget that holds two jobs."""
+    import rtm_amd
+    print("case: interleaved submissions under a two-job budget", flush=True)
+    with rtm_amd.BatchEngine(0) as probe:
+        sizes = [probe.job_bytes(m, p, nt, dt, s, f) for (m, p, nt, dt, s, f) in jobs]
+    budget = sorted(sizes)[-1] + sorted(sizes)[-2]
+    with rtm_amd.BatchEngine(0, memory_budget=budget) as b:
+        ids = {}
+        for k in range(3):
+            ids[k] = b.submit(*jobs[k][:5], snapshot_format=jobs[k][5])
+        b.run(max_ticks=60, until_done=False)
+        for k in range(3, len(jobs)):
+            ids[k] = b.submit(*jobs[k][:5], snapshot_format=jobs[k][5])
+            b.run(max_ticks=25, until_done=False)
+            assert b.stats()["bytes_in_use"] <= budget
+        b.run()
+        for k, jid in ids.items():
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}"
+            assert bits_equal(alone[k][1], S), f"job {k}"
+
+
+def test_fp64_stack_at_offsets(jobs, alone):
+    import rtm_amd
+    print("case: FP64 stack of the batch at per-job offsets", flush=True)
+    NX, NZ = 160, 90
+    want_i = np.zeros((NX, NZ))
+    want_s = np.zeros((NX, NZ))
+    with rtm_amd.BatchEngine(0) as b:
+        b.set_stack(NX, NZ)
+        for k, (m, p, nt, dt, s, f) in enumerate(jobs):
+            ix0, iz0 = (7 * k) % (NX - m.nx + 1), (5 * k) % (NZ - m.nz + 1)
+            b.submit(m, p, nt, dt, s, snapshot_format=f, stack_at=(ix0, iz0))
+            want_i[ix0:ix0 + m.nx, iz0:iz0 + m.nz] += alone[k][0].reshape(m.nx, m.nz).astype(np.float64)
+            want_s[ix0:ix0 + m.nx, iz0:iz0 + m.nz] += alone[k][1].reshape(m.nx, m.nz).astype(np.float64)
+        b.run()
+        gi, gs = b.stack()
+    # FP64 sums of the same FP32 addends; only the order of addition may differ
+    np.testing.assert_allclose(gi.reshape(NX, NZ), want_i, rtol=1e-12, atol=1e-12 * np.abs(want_i).max())
+    np.testing.assert_allclose(gs.reshape(NX, NZ), want_s, rtol=1e-12, atol=1e-12 * np.abs(want_s).max())
+
+
+def test_batch_errors(jobs):
+    import rtm_amd
+    print("case: batch argument errors", flush=True)
+    m, p, nt, dt, s, f = jobs[0]
+    with rtm_amd.BatchEngine(0, memory_budget=1 << 20) as b:
+        with pytest.raises(RuntimeError, match="budget"):
+            b.submit(m, p, nt, dt, s)
+    with rtm_amd.BatchEngine(0) as b:
+        bad = rtm_amd.RTMParams(nb=p.nb, order=7, store_interval=1)
+        with pytest.raises(RuntimeError, match="unsupported spatial order"):
+            b.submit(m, bad, nt, dt, s)
+        with pytest.raises(RuntimeError, match="fewer than nt"):
+            b.submit(m, p, s.nt + 1, dt, s)
+        with pytest.raises(RuntimeError, match="no stack is set"):
+            b.submit(m, p, nt, dt, s, stack_at=(0, 0))
+        b.set_stack(m.nx, m.nz)
+        with pytest.raises(RuntimeError, match="does not fit"):
+            b.submit(m, p, nt, dt, s, stack_at=(1, 0))
+        jid = b.submit(m, p, nt, dt, s)
+        with pytest.raises(RuntimeError, match="not finished"):
+            b.result(jid)
+        b.run()
+        b.result(jid)
+        with pytest.raises(RuntimeError, match="unknown job"):
+            b.state(jid)

@@ -1,0 +1,59 @@
+"""The command-line drop-ins on the GPU, end to end, against the reference's
+own outputs: rtm_synth's built-in model and the layered workload give the
+reference's RTMS bytes; rtm --engine hip gives the reference image and
+illumination map bit for bit; rtm_compare passes it."""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(REPO, "rtm-acceleration_amd", "bin")
+GOLD = os.path.join(REPO, "tests", "golden")
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def run(*args):
+    print("case: cli", " ".join(args[:2]), flush=True)
+    r = subprocess.run([os.path.join(BIN, args[0]), *args[1:]], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r
+
+
+def test_rtm_synth_builtin_model(tmp_path):
+    man = json.load(open(os.path.join(GOLD, "builtin", "manifest.json")))
+    run("rtm_synth", "--outdir", str(tmp_path), *man["args"])
+    assert sha(tmp_path / "velocity.bin") == man["sha256"]["velocity.bin"]
+    assert sha(tmp_path / "shots.bin") == man["sha256"]["shots.bin"]
+
+
+def test_layered3_cli_pipeline(tmp_path):
+    d = os.path.join(GOLD, "layered3")
+    man = json.load(open(os.path.join(d, "manifest.json")))
+    vel = os.path.join(d, "velocity.bin")
+    shots = tmp_path / "shots.bin"
+    run("rtm_synth", "--velocity", vel, "--outdir", str(tmp_path), "--shots-out", str(shots), "--nt", "1400",
+        "--dt", "0.001", "--f0", "12", "--nshots", "4", "--rec-dx", "10")
+    assert sha(shots) == man["sha256"]["shots.rtms"]
+    out = tmp_path / "image.bin"
+    csv = tmp_path / "bench.csv"
+    r = run("rtm", "--engine", "hip", "--velocity", vel, "--shots", str(shots), "--output", str(out),
+            "--order", "8", "--nb", "60", "--f0", "12", "--store-interval", "2", "--mute-direct", "1500",
+            "--write-illum-map", "--illumination", "--filter", "laplacian", "--benchmark-csv", str(csv),
+            "--quiet")
+    assert "Cell rate" in r.stdout
+    assert sha(out) == man["sha256"]["image.f32"]
+    assert sha(tmp_path / "image_illummap.bin") == man["sha256"]["illummap.f32"]
+    assert (tmp_path / "image_illum.bin").stat().st_size == 201 * 151 * 4
+    assert (tmp_path / "image_filtered.bin").stat().st_size == 201 * 151 * 4
+    assert csv.read_text().splitlines()[0].count(",") == 25
+    run("rtm_compare", os.path.join(d, "image.f32"), str(out), "--nx", "201", "--nz", "151")
