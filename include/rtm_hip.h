@@ -254,6 +254,7 @@ typedef struct {
     double    cell_steps;        /* 2*nt*nxe*nze summed over finished jobs          */
     double    run_seconds;       /* wall time inside rtm_hip_batch_run             */
     double    busy_latency_s;    /* sum of submit-to-done times of finished jobs   */
+    int       models_resident;   /* registered models (rtm_hip_batch_add_model)     */
     size_t    bytes_in_use, bytes_budget;
     int       ngroups;           /* stencil half-widths with a launch per tick     */
     int       group_half[6], group_tile_x[6], group_dpp[6];
@@ -266,11 +267,29 @@ int         rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rt
 void        rtm_hip_batch_destroy(rtm_hip_batch* b);
 const char* rtm_hip_batch_last_error(const rtm_hip_batch* b);
 
-/* Device bytes the job would hold while active (admission figure). */
+/* Replaces RTMEngine::setup (include/rtm_engine.hpp:25-27) for the batch:
+ * validates the model, parameters and time axis as setup does, builds the
+ * tables on the host with the reference's float formulas and uploads them
+ * now.  The model stays resident (and counts against the budget) until
+ * removed, and serves every shot submitted on it; params->snapshot_format
+ * applies to those shots. */
+int         rtm_hip_batch_add_model(rtm_hip_batch* b, const rtm_hip_grid* grid, const float* v,
+                                    const rtm_hip_params* params, int nt, float dt, int* model_id);
+/* Forgets the model; its tables are freed once no queued or active job uses it. */
+int         rtm_hip_batch_remove_model(rtm_hip_batch* b, int model_id);
+
+/* Queues one shot on a resident model (the per-shot body of RTMEngine::migrate,
+ * src/rtm/rtm.cpp:88-101); stack position as in rtm_hip_job. */
+int         rtm_hip_batch_submit_shot(rtm_hip_batch* b, int model_id, const rtm_hip_shot* shot,
+                                      int stack_ix0, int stack_iz0, long long* id);
+
+/* Device bytes admission needs for the job (its own buffers plus the tables
+ * of its private model). */
 int         rtm_hip_batch_job_bytes(rtm_hip_batch* b, const rtm_hip_job* job, size_t* bytes);
 
-/* Validates and queues a job (inputs copied; the caller's buffers are free on
- * return) and returns its id.  Nothing runs until rtm_hip_batch_run. */
+/* Validates and queues a job with a private model (inputs copied; the
+ * caller's buffers are free on return) and returns its id.  Nothing runs
+ * until rtm_hip_batch_run. */
 int         rtm_hip_batch_submit(rtm_hip_batch* b, const rtm_hip_job* job, long long* id);
 
 /* Admits queued jobs and runs ticks.  until_done = 1: until no job is queued
@@ -286,7 +305,8 @@ int         rtm_hip_batch_job_state(rtm_hip_batch* b, long long id, int* state, 
  * finished job; the job is forgotten afterwards. */
 int         rtm_hip_batch_result(rtm_hip_batch* b, long long id, float* image, float* illum);
 
-/* Allocates (and zeroes) an nx*nz FP64 image/illumination stack. */
+/* Allocates (and zeroes) an nx*nz FP64 image/illumination stack; refused
+ * while queued or active jobs have a stack position. */
 int         rtm_hip_batch_set_stack(rtm_hip_batch* b, int nx, int nz);
 int         rtm_hip_batch_get_stack(rtm_hip_batch* b, double* image, double* illum);
 

@@ -2,22 +2,29 @@
 // rtm_hip_batch_*): many shot-migration jobs, each on its own grid, advance
 // together one time step per tick (batch_kernels.hip).
 //
-// Scheduling.  Submitted jobs wait in a FIFO queue.  A job is admitted when
-// its device footprint (tables, three wavefields, snapshot store, image,
-// traces, injection tables) fits in what the memory budget has left and the
-// active-job cap allows; admission allocates and uploads it and the launch
-// tables are rebuilt.  The engine then runs ticks until the next job
-// finishes (the host knows every job's clock: one step per tick), takes the
-// finished job's image (and adds it into the FP64 stack when the job has a
-// stack position), frees its memory and admits from the queue again: jobs
-// that arrive while others are on the device start as soon as memory frees
-// up, and their steps overlap with the running jobs' steps in the same
-// launches (continuous batching).
+// Models.  A model is what RTMEngine::setup binds (velocity grid, RTMParams,
+// time axis): its tables ((v dt)^2, tapers, wavelet, stencil coefficients)
+// are built on the host with the reference's float formulas and uploaded
+// once.  A registered model stays resident on the device until it is removed
+// and serves any number of shots; a job submitted with its own model gets a
+// private one that lives as long as the job.
+//
+// Scheduling.  Submitted jobs wait in a FIFO queue.  The head job is admitted
+// when its device footprint (three wavefields, snapshot store, image, traces,
+// injection tables, plus its model's tables when they are not resident yet)
+// fits in what the memory budget has left and the active-job cap allows;
+// admission allocates and uploads it and the launch tables are rebuilt.  The
+// engine then runs ticks until the next job finishes (the host knows every
+// job's clock: one step per tick), takes the finished job's image (and adds
+// it into the FP64 stack when the job has a stack position), frees its
+// memory and admits from the queue again: jobs that arrive while others are
+// on the device start as soon as memory frees up, and their steps overlap
+// with the running jobs' steps in the same launches (continuous batching).
 //
 // Launches.  Jobs are grouped by stencil half-width; each group has one
 // launch per tick over the concatenation of its jobs' tiles, with the tile
 // width and cross-lane method picked once per half-width by timing dry runs
-// on the live jobs.  A tick (the group launches plus the clock kernel) can be
+// on the live jobs.  A tick (the group launches plus the clock kernel) is
 // captured once in a hipGraph and replayed until the job set changes.
 #include <hip/hip_runtime.h>
 
@@ -50,37 +57,45 @@ double now_s() {
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-// Host copy of one submitted job and, once admitted, its device buffers.
-struct Job {
-    long long id = 0;
-    int state = RTM_HIP_JOB_QUEUED;
-    // inputs
+// One set-up model: grid, parameters, time axis and (when resident) its
+// device tables.
+struct Model {
+    int id = 0;
+    bool registered = false;         // stays resident until removed
+    bool removing = false;           // removal requested while jobs still use it
+    int refs = 0;                    // queued + active jobs on it
     rtm_hip_grid grid{};
-    std::vector<float> v;
     rtm_hip_params params{};
     int nt = 0;
     float dt = 0.0f;
+    std::vector<float> v;            // host velocity until the tables are uploaded
+    int half = 0, nxe = 0, nze = 0, ldz = 0, nsnap = 0;
+    size_t n_int = 0, n_fld = 0, bytes = 0;
+    rtm_hip::Coeffs coef{};
+    bool resident = false;
+    float *vdt2 = nullptr, *wx = nullptr, *wz = nullptr, *wavelet = nullptr;
+};
+
+// One submitted shot migration and, once admitted, its device buffers.
+struct Job {
+    long long id = 0;
+    int state = RTM_HIP_JOB_QUEUED;
+    std::shared_ptr<Model> model;
     float sx = 0.0f, sz = 0.0f;
     std::vector<float> rx, rz, traces;
     int trace_nt = 0;
     int stack_ix0 = -1, stack_iz0 = -1;
-    // derived
-    int half = 0, nxe = 0, nze = 0, ldz = 0, nsnap = 0;
-    size_t n_int = 0, n_fld = 0, bytes = 0;
-    rtm_hip::Coeffs coef{};
-    // device
+    size_t bytes = 0;                // device bytes of the job itself (model excluded)
     std::vector<void*> allocs;       // freed when the job retires
     std::vector<void*> entry_allocs; // injection tables of the current tile width
     JobDesc desc{};
-    int tile_x_built = 0;            // geometry the entry tables were built for
-    int steps_left = 0;              // ticks until done (host mirror of the clock)
-    // timing and result
+    int tile_x_built = 0;            // tile width the injection tables were built for
+    int steps_left = 0;              // ticks until done (host mirror of the device clock)
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::vector<float> image, illum;
 };
 
 struct Group {
-    int half = 0;
     int tile_x = 4;
     bool dpp = true;
     bool tuned = false;
@@ -97,21 +112,21 @@ struct rtm_hip_batch {
     std::string err;
     rtm_hip_batch_config cfg{};
     size_t budget = 0, used = 0;
-    long long next_id = 1;
+    long long next_job = 1;
+    int next_model = 1;
+    std::map<int, std::shared_ptr<Model>> models;    // registered
     std::map<long long, std::unique_ptr<Job>> jobs;
     std::deque<Job*> queue;
     std::vector<Job*> active;
     JobDesc* d_jobs = nullptr;
     size_t jobs_cap = 0;
-    std::map<int, Group> groups;     // by half-width
+    std::map<int, Group> groups;                     // by stencil half-width
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // FP64 stack
     int stack_nx = 0, stack_nz = 0;
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
-    // accounting
     rtm_hip_batch_stats stats{};
 
     int fail(int code, const char* fmt, ...) {
@@ -137,55 +152,135 @@ struct rtm_hip_batch {
 
 namespace {
 
-// Device bytes one job needs while active.
-size_t job_footprint(const Job& j) {
-    const size_t snap_elem = j.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
-    const size_t tiles_bound = (size_t)(j.nxe / 4 + 2) * (j.nze / 52 + 2) + 1;   // tile_x >= 4, strip >= 52
-    size_t b = 0;
-    b += 4 * j.n_fld * sizeof(float);                       // vdt2 + three wavefields
-    b += ((size_t)j.nxe + j.ldz + j.nt) * sizeof(float);    // tapers, wavelet
-    b += (size_t)j.nsnap * j.n_int * snap_elem;             // snapshot store
-    b += (size_t)j.nsnap * sizeof(float);                   // snapshot scales
-    b += 2 * j.n_int * sizeof(float);                       // image, illumination
-    b += j.traces.size() * sizeof(float);                   // receiver data
-    b += 2 * (tiles_bound * sizeof(int) + (j.rx.size() + 1) * sizeof(Entry));
-    return b;
-}
+constexpr double kGiB = 1073741824.0;
 
-int derive(rtm_hip_batch* b, Job& j) {
-    const int order = j.params.order;
+// ---- models ---------------------------------------------------------------
+
+// Validates the setup arguments (the checks and messages of
+// CPUReferenceRTM::setup, src/rtm/rtm_cpu.cpp:9-31) and derives the sizes.
+int model_init(rtm_hip_batch* b, Model& m, const rtm_hip_grid* grid, const float* v, const rtm_hip_params* params,
+               int nt, float dt) {
+    if (!grid || !v || !params) return b->fail(RTM_HIP_ERR_ARG, "null model argument");
+    m.grid = *grid;
+    m.params = *params;
+    m.nt = nt;
+    m.dt = dt;
+    const int order = m.params.order;
     if (order != 2 && order != 4 && order != 6 && order != 8 && order != 10 && order != 12)
         return b->fail(RTM_HIP_ERR_ARG, "unsupported spatial order (use 2, 4, 6, 8, 10 or 12)");
-    j.half = order / 2;
-    if (j.params.nb < j.half + 1) return b->fail(RTM_HIP_ERR_ARG, "--nb must be at least order/2 + 1");
-    if (j.grid.nx <= 0 || j.grid.nz <= 0) return b->fail(RTM_HIP_ERR_ARG, "velocity model size does not match nx*nz");
-    if (j.params.store_interval < 1) return b->fail(RTM_HIP_ERR_ARG, "--store-interval must be >= 1");
-    if (j.nt < 1 || !(j.dt > 0.0f)) return b->fail(RTM_HIP_ERR_ARG, "nt must be >= 1 and dt > 0");
-    const int fmt = j.params.snapshot_format;
+    m.half = order / 2;
+    if (m.params.nb < m.half + 1) return b->fail(RTM_HIP_ERR_ARG, "--nb must be at least order/2 + 1");
+    if (m.grid.nx <= 0 || m.grid.nz <= 0) return b->fail(RTM_HIP_ERR_ARG, "velocity model size does not match nx*nz");
+    if (m.params.store_interval < 1) return b->fail(RTM_HIP_ERR_ARG, "--store-interval must be >= 1");
+    if (m.nt < 1 || !(m.dt > 0.0f)) return b->fail(RTM_HIP_ERR_ARG, "nt must be >= 1 and dt > 0");
+    const int fmt = m.params.snapshot_format;
     if (fmt != RTM_HIP_SNAP_F32 && fmt != RTM_HIP_SNAP_F16 && fmt != RTM_HIP_SNAP_BF16)
         return b->fail(RTM_HIP_ERR_ARG, "unknown snapshot format");
-    if (j.rx.empty()) return b->fail(RTM_HIP_ERR_ARG, "a migration job needs receivers and traces");
-    if (j.trace_nt < j.nt)
-        return b->fail(RTM_HIP_ERR_ARG, "shot has %d samples per trace, fewer than nt=%d", j.trace_nt, j.nt);
-    j.nxe = j.grid.nx + 2 * j.params.nb;
-    j.nze = j.grid.nz + 2 * j.params.nb;
-    j.ldz = round_up(j.nze, rtm_hip::kTileZ);
-    if ((long long)j.nxe * j.ldz >= (1LL << 31)) return b->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
-    j.nsnap = (j.nt - 1) / j.params.store_interval + 1;
-    j.n_int = (size_t)j.grid.nx * j.grid.nz;
-    j.n_fld = (size_t)j.nxe * j.ldz;
-    j.bytes = job_footprint(j);
+    m.nxe = m.grid.nx + 2 * m.params.nb;
+    m.nze = m.grid.nz + 2 * m.params.nb;
+    m.ldz = round_up(m.nze, rtm_hip::kTileZ);
+    if ((long long)m.nxe * m.ldz >= (1LL << 31)) return b->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
+    m.nsnap = (m.nt - 1) / m.params.store_interval + 1;
+    m.n_int = (size_t)m.grid.nx * m.grid.nz;
+    m.n_fld = (size_t)m.nxe * m.ldz;
+    m.bytes = (m.n_fld + (size_t)m.nxe + (size_t)m.ldz + (size_t)m.nt) * sizeof(float);
+    m.v.assign(v, v + m.n_int);
     return RTM_HIP_OK;
 }
 
+void model_release(Model& m) {
+    for (float* p : {m.vdt2, m.wx, m.wz, m.wavelet})
+        if (p) (void)hipFree(p);
+    m.vdt2 = m.wx = m.wz = m.wavelet = nullptr;
+    m.resident = false;
+}
+
+int model_upload(rtm_hip_batch* b, Model& m) {
+    rtm_hip::HostTables t;
+    try {
+        t = rtm_hip::build_tables(m.grid.nx, m.grid.nz, m.grid.dx, m.grid.dz, m.params.nb, m.params.order,
+                                  m.params.sponge_alpha, m.params.f0, m.nt, m.dt, m.v.data(), m.ldz);
+    } catch (const std::exception& ex) {
+        return b->fail(RTM_HIP_ERR_ARG, "%s", ex.what());
+    }
+    m.coef = t.coef;
+    struct Up {
+        float** dst;
+        const std::vector<float>* src;
+    } ups[] = {{&m.vdt2, &t.vdt2}, {&m.wx, &t.wx}, {&m.wz, &t.wz}, {&m.wavelet, &t.wavelet}};
+    for (const Up& u : ups) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(u.dst), std::max<size_t>(u.src->size(), 1) * sizeof(float));
+        if (e == hipSuccess)
+            e = hipMemcpy(*u.dst, u.src->data(), u.src->size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            model_release(m);
+            return b->hip_fail(e, "model tables");
+        }
+    }
+    m.resident = true;
+    m.v.clear();
+    m.v.shrink_to_fit();
+    b->used += m.bytes;
+    return RTM_HIP_OK;
+}
+
+// Frees a model's tables once nothing needs them any more.
+void model_maybe_free(rtm_hip_batch* b, const std::shared_ptr<Model>& m) {
+    if (m->refs > 0 || !m->resident) return;
+    if (m->registered && !m->removing) return;
+    model_release(*m);
+    b->used -= m->bytes;
+}
+
+// ---- jobs -----------------------------------------------------------------
+
+size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
+    const size_t snap_elem = m.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
+    const size_t tiles_bound = (size_t)(m.nxe / 4 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 4, strip >= 52
+    size_t bytes = 0;
+    bytes += 3 * m.n_fld * sizeof(float);                   // three wavefields
+    bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store
+    bytes += (size_t)m.nsnap * sizeof(float);               // snapshot scales
+    bytes += 2 * m.n_int * sizeof(float);                   // image, illumination
+    bytes += ntraces * sizeof(float);                       // receiver data
+    bytes += 2 * (tiles_bound * sizeof(int) + (nrec + 1) * sizeof(Entry));
+    return bytes;
+}
+
+int job_init(rtm_hip_batch* b, Job& j, const std::shared_ptr<Model>& m, const rtm_hip_shot* shot, int ix0, int iz0) {
+    if (!shot || !shot->rx || !shot->rz || !shot->traces || shot->nrec <= 0)
+        return b->fail(RTM_HIP_ERR_ARG, "a migration job needs receivers and traces");
+    if (shot->nt < m->nt)
+        return b->fail(RTM_HIP_ERR_ARG, "shot has %d samples per trace, fewer than nt=%d", shot->nt, m->nt);
+    if (ix0 >= 0) {
+        if (!b->d_stack_i) return b->fail(RTM_HIP_ERR_ARG, "job has a stack position but no stack is set");
+        if (iz0 < 0 || ix0 + m->grid.nx > b->stack_nx || iz0 + m->grid.nz > b->stack_nz)
+            return b->fail(RTM_HIP_ERR_ARG, "job image does not fit in the stack at its position");
+    }
+    j.model = m;
+    j.sx = shot->sx;
+    j.sz = shot->sz;
+    j.rx.assign(shot->rx, shot->rx + shot->nrec);
+    j.rz.assign(shot->rz, shot->rz + shot->nrec);
+    j.trace_nt = shot->nt;
+    j.traces.assign(shot->traces, shot->traces + (size_t)shot->nrec * shot->nt);
+    j.stack_ix0 = ix0;
+    j.stack_iz0 = ix0 >= 0 ? iz0 : -1;
+    j.bytes = job_footprint(*m, j.rx.size(), j.traces.size());
+    return RTM_HIP_OK;
+}
+
+// Device bytes admission needs for the job now.
+size_t admission_bytes(const Job& j) { return j.bytes + (j.model->resident ? 0 : j.model->bytes); }
+
 template <typename T>
-int job_alloc(rtm_hip_batch* b, Job& j, T** p, size_t count, const char* what) {
+int job_alloc(rtm_hip_batch* b, std::vector<void*>& list, T** p, size_t count, const char* what) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
         return b->hip_fail(e, what);
     }
-    j.allocs.push_back(*p);
+    list.push_back(*p);
     return RTM_HIP_OK;
 }
 
@@ -203,19 +298,20 @@ void job_free(Job& j) {
 
 // Nearest-grid-point cell of a position, clamped to the interior
 // (src/rtm/rtm_cpu.cpp:99-123).
-void job_cell(const Job& j, float x, float z, int* ixe, int* ize) {
-    const int ix0 = static_cast<int>(std::lround((x - j.grid.ox) / j.grid.dx));
-    const int iz0 = static_cast<int>(std::lround((z - j.grid.oz) / j.grid.dz));
-    *ixe = std::clamp(ix0, 0, j.grid.nx - 1) + j.params.nb;
-    *ize = std::clamp(iz0, 0, j.grid.nz - 1) + j.params.nb;
+void job_cell(const Model& m, float x, float z, int* ixe, int* ize) {
+    const int ix0 = static_cast<int>(std::lround((x - m.grid.ox) / m.grid.dx));
+    const int iz0 = static_cast<int>(std::lround((z - m.grid.oz) / m.grid.dz));
+    *ixe = std::clamp(ix0, 0, m.grid.nx - 1) + m.params.nb;
+    *ize = std::clamp(iz0, 0, m.grid.nz - 1) + m.params.nb;
 }
 
 // Per-tile injection tables of the job for tile width tile_x (forward: the
-// source; backward: every receiver, in receiver order per cell).
+// source; backward: every receiver, in receiver order within a cell).
 int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
     free_entries(j);
+    const Model& m = *j.model;
     const rtm_hip::StepGeometry g =
-        rtm_hip::step_geometry(rtm_hip::Family::kMarchDpp, j.half, tile_x, j.nxe, j.nze, j.ldz);
+        rtm_hip::step_geometry(rtm_hip::Family::kMarchDpp, m.half, tile_x, m.nxe, m.nze, m.ldz);
     for (int k = 0; k < 2; ++k) {
         struct T {
             int tile, cell, code;
@@ -223,12 +319,12 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
         std::vector<T> v;
         int ixe, ize;
         if (k == 0) {
-            job_cell(j, j.sx, j.sz, &ixe, &ize);
-            v.push_back({rtm_hip::tile_of_cell(g, ixe, ize), ixe * j.ldz + ize, -1});
+            job_cell(m, j.sx, j.sz, &ixe, &ize);
+            v.push_back({rtm_hip::tile_of_cell(g, ixe, ize), ixe * m.ldz + ize, -1});
         } else {
             for (size_t ir = 0; ir < j.rx.size(); ++ir) {
-                job_cell(j, j.rx[ir], j.rz[ir], &ixe, &ize);
-                v.push_back({rtm_hip::tile_of_cell(g, ixe, ize), ixe * j.ldz + ize, (int)ir});
+                job_cell(m, j.rx[ir], j.rz[ir], &ixe, &ize);
+                v.push_back({rtm_hip::tile_of_cell(g, ixe, ize), ixe * m.ldz + ize, (int)ir});
             }
         }
         std::sort(v.begin(), v.end(), [](const T& a, const T& c) {
@@ -245,12 +341,8 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
         for (int t = 0; t < g.ntiles; ++t) ptr[t + 1] += ptr[t];
         int* d_ptr = nullptr;
         Entry* d_ent = nullptr;
-        if (int rc = job_alloc(b, j, &d_ptr, ptr.size(), "injection tables")) return rc;
-        if (int rc = job_alloc(b, j, &d_ent, ent.size(), "injection tables")) return rc;
-        j.allocs.pop_back();
-        j.allocs.pop_back();
-        j.entry_allocs.push_back(d_ptr);
-        j.entry_allocs.push_back(d_ent);
+        if (int rc = job_alloc(b, j.entry_allocs, &d_ptr, ptr.size(), "injection tables")) return rc;
+        if (int rc = job_alloc(b, j.entry_allocs, &d_ent, ent.size(), "injection tables")) return rc;
         hipError_t e = hipMemcpy(d_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess && !ent.empty())
             e = hipMemcpy(d_ent, ent.data(), ent.size() * sizeof(Entry), hipMemcpyHostToDevice);
@@ -265,61 +357,46 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
     return RTM_HIP_OK;
 }
 
-// Allocate, build and upload everything an admitted job needs.
+// Allocate, zero and upload everything an admitted job needs.
 int admit_job(rtm_hip_batch* b, Job& j) {
-    rtm_hip::HostTables t;
-    try {
-        t = rtm_hip::build_tables(j.grid.nx, j.grid.nz, j.grid.dx, j.grid.dz, j.params.nb, j.params.order,
-                                  j.params.sponge_alpha, j.params.f0, j.nt, j.dt, j.v.data(), j.ldz);
-    } catch (const std::exception& ex) {
-        return b->fail(RTM_HIP_ERR_ARG, "%s", ex.what());
-    }
+    Model& m = *j.model;
+    if (!m.resident)
+        if (int rc = model_upload(b, m)) return rc;
     JobDesc& d = j.desc;
     d = JobDesc{};
-    d.nxe = j.nxe; d.nze = j.nze; d.ldz = j.ldz; d.nb = j.params.nb; d.nx = j.grid.nx; d.nz = j.grid.nz;
-    d.half = j.half; d.nt = j.nt; d.store = j.params.store_interval; d.snap_fmt = j.params.snapshot_format;
-    d.coef = t.coef;
+    d.nxe = m.nxe; d.nze = m.nze; d.ldz = m.ldz; d.nb = m.params.nb; d.nx = m.grid.nx; d.nz = m.grid.nz;
+    d.half = m.half; d.nt = m.nt; d.store = m.params.store_interval; d.snap_fmt = m.params.snapshot_format;
+    d.coef = m.coef;
     d.trace_stride = j.trace_nt;
-    d.n_int = (long long)j.n_int;
-    d.bwd_base = (j.nt + 1) % 3;            // buffer of the forward phase's last field
-    float *vdt2, *wx, *wz, *wav, *tr, *img, *ill;
-    int rc = RTM_HIP_OK;
-    if (!rc) rc = job_alloc(b, j, &vdt2, j.n_fld, "vdt2");
-    if (!rc) rc = job_alloc(b, j, &wx, (size_t)j.nxe, "wx");
-    if (!rc) rc = job_alloc(b, j, &wz, (size_t)j.ldz, "wz");
-    if (!rc) rc = job_alloc(b, j, &wav, (size_t)j.nt, "wavelet");
-    if (!rc) rc = job_alloc(b, j, &tr, j.traces.size(), "traces");
-    if (!rc) rc = job_alloc(b, j, &img, j.n_int, "image");
-    if (!rc) rc = job_alloc(b, j, &ill, j.n_int, "illumination");
-    for (int k = 0; k < 3 && !rc; ++k) rc = job_alloc(b, j, &d.fld[k], j.n_fld, "wavefield");
+    d.n_int = (long long)m.n_int;
+    d.bwd_base = (m.nt + 1) % 3;            // buffer of the forward phase's last field
+    d.vdt2 = m.vdt2; d.wx = m.wx; d.wz = m.wz; d.wavelet = m.wavelet;
+    float *tr = nullptr, *img = nullptr, *ill = nullptr;
+    int rc = job_alloc(b, j.allocs, &tr, j.traces.size(), "traces");
+    if (!rc) rc = job_alloc(b, j.allocs, &img, m.n_int, "image");
+    if (!rc) rc = job_alloc(b, j.allocs, &ill, m.n_int, "illumination");
+    for (int k = 0; k < 3 && !rc; ++k) rc = job_alloc(b, j.allocs, &d.fld[k], m.n_fld, "wavefield");
     if (d.snap_fmt == RTM_HIP_SNAP_F32) {
-        if (!rc) rc = job_alloc(b, j, &d.snap32, (size_t)j.nsnap * j.n_int, "snapshot store");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.snap32, (size_t)m.nsnap * m.n_int, "snapshot store");
     } else {
-        if (!rc) rc = job_alloc(b, j, &d.snap16, (size_t)j.nsnap * j.n_int, "snapshot store");
-        if (!rc) rc = job_alloc(b, j, &d.snap_max, (size_t)j.nsnap, "snapshot scales");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.snap16, (size_t)m.nsnap * m.n_int, "snapshot store");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.snap_max, (size_t)m.nsnap, "snapshot scales");
     }
     if (rc) return rc;
-    hipError_t e = hipMemcpy(vdt2, t.vdt2.data(), t.vdt2.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(wx, t.wx.data(), t.wx.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(wz, t.wz.data(), t.wz.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(wav, t.wavelet.data(), t.wavelet.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(tr, j.traces.data(), j.traces.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(img, 0, j.n_int * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(ill, 0, j.n_int * sizeof(float));
-    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(d.fld[k], 0, j.n_fld * sizeof(float));
-    if (e == hipSuccess && d.snap_max) e = hipMemset(d.snap_max, 0, (size_t)j.nsnap * sizeof(float));
+    hipError_t e = hipMemcpy(tr, j.traces.data(), j.traces.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(img, 0, m.n_int * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(ill, 0, m.n_int * sizeof(float));
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(d.fld[k], 0, m.n_fld * sizeof(float));
+    if (e == hipSuccess && d.snap_max) e = hipMemset(d.snap_max, 0, (size_t)m.nsnap * sizeof(float));
     if (e != hipSuccess) return b->hip_fail(e, "job upload");
-    d.vdt2 = vdt2; d.wx = wx; d.wz = wz; d.wavelet = wav; d.traces = tr; d.image = img; d.illum = ill;
-    d.phase = 0;
-    d.step = 0;
-    j.steps_left = 2 * j.nt;
-    j.tile_x_built = 0;
+    d.traces = tr; d.image = img; d.illum = ill;
+    j.steps_left = 2 * m.nt;
     j.traces.clear();
     j.traces.shrink_to_fit();
-    j.v.clear();
-    j.v.shrink_to_fit();
     return RTM_HIP_OK;
 }
+
+// ---- launch tables, ticks, autotuning ------------------------------------
 
 // The launch tables of every half-width group and the device descriptors.
 int rebuild(rtm_hip_batch* b) {
@@ -335,14 +412,11 @@ int rebuild(rtm_hip_batch* b) {
         }
     }
     std::map<int, std::vector<int>> by_half;
-    for (size_t k = 0; k < b->active.size(); ++k) by_half[b->active[k]->half].push_back((int)k);
-    for (auto it = b->groups.begin(); it != b->groups.end();) {
-        if (!by_half.count(it->first)) it->second.ntiles = 0;
-        ++it;
-    }
+    for (size_t k = 0; k < b->active.size(); ++k) by_half[b->active[k]->model->half].push_back((int)k);
+    for (auto& [half, g] : b->groups)
+        if (!by_half.count(half)) g.ntiles = 0;
     for (auto& [half, idx] : by_half) {
         Group& g = b->groups[half];
-        g.half = half;
         std::vector<int> tile_job;
         for (int k : idx) {
             Job& j = *b->active[k];
@@ -365,12 +439,13 @@ int rebuild(rtm_hip_batch* b) {
         hipError_t e = hipMemcpy(g.d_tile_job, tile_job.data(), tile_job.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e != hipSuccess) return b->hip_fail(e, "tile table upload");
     }
-    // the host knows every clock: one step per tick since admission
+    // every clock is known on the host: one step per tick since admission
     std::vector<JobDesc> descs;
     for (Job* j : b->active) {
-        const int done = 2 * j->nt - j->steps_left;
-        j->desc.phase = done / j->nt;
-        j->desc.step = done % j->nt;
+        const int nt = j->model->nt;
+        const int done = 2 * nt - j->steps_left;
+        j->desc.phase = done / nt;
+        j->desc.step = done % nt;
         descs.push_back(j->desc);
     }
     if (!descs.empty()) {
@@ -382,47 +457,52 @@ int rebuild(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
-// Launches of one tick.
-int enqueue_tick(rtm_hip_batch* b, int dry) {
+int enqueue_tick(rtm_hip_batch* b) {
     for (auto& [half, g] : b->groups) {
         if (g.ntiles == 0) continue;
-        hipError_t e = rtm_hip::launch_batch_step(half, g.tile_x, g.dpp, b->d_jobs, g.d_tile_job, g.ntiles, dry,
+        hipError_t e = rtm_hip::launch_batch_step(half, g.tile_x, g.dpp, b->d_jobs, g.d_tile_job, g.ntiles, 0,
                                                   b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
     }
-    if (dry) return RTM_HIP_OK;
     hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), b->stream);
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Pick tile width and cross-lane method per half-width by timing dry ticks
-// (no clock advance, no image or snapshot update) on the live jobs.
+// Tile width and cross-lane method per half-width, picked by timing dry
+// steps (no clock advance, no image or snapshot update: the next real step
+// rewrites the same p^{n+1}) on the live jobs; a candidate must beat the
+// best by 3 % to replace it.
 int autotune(rtm_hip_batch* b) {
     bool any = false;
-    for (auto& [half, g] : b->groups)
-        if (g.ntiles > 0 && !g.tuned) any = true;
+    for (auto& [half, g] : b->groups) any |= g.ntiles > 0 && !g.tuned;
     if (!any) return RTM_HIP_OK;
-    hipEvent_t t0, t1;
-    (void)hipEventCreate(&t0);
-    (void)hipEventCreate(&t1);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipError_t e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
     for (auto& [half, g] : b->groups) {
-        if (g.ntiles == 0 || g.tuned) continue;
+        if (rc || g.ntiles == 0 || g.tuned) continue;
         float best = 1e30f;
         int best_tx = g.tile_x;
         bool best_dpp = g.dpp;
         for (int tx : {4, 8, 16}) {
             for (bool dpp : {true, false}) {
+                if (rc) break;
                 g.tile_x = tx;
                 g.dpp = dpp;
-                if (int rc = rebuild(b)) return rc;
-                for (int rep = 0; rep < 4; ++rep) {
-                    if (rep == 1) (void)hipEventRecord(t0, b->stream);
-                    hipError_t e = rtm_hip::launch_batch_step(half, tx, dpp, b->d_jobs, g.d_tile_job, g.ntiles, 1,
-                                                              b->stream);
-                    if (e != hipSuccess) return b->hip_fail(e, "autotune launch");
+                if ((rc = rebuild(b))) break;
+                for (int rep = 0; rep < 6 && !rc; ++rep) {
+                    if (rep == 2) (void)hipEventRecord(t0, b->stream);
+                    e = rtm_hip::launch_batch_step(half, tx, dpp, b->d_jobs, g.d_tile_job, g.ntiles, 1, b->stream);
+                    if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                 }
+                if (rc) break;
                 (void)hipEventRecord(t1, b->stream);
-                (void)hipEventSynchronize(t1);
+                e = hipEventSynchronize(t1);
+                if (e != hipSuccess) {
+                    rc = b->hip_fail(e, "autotune");
+                    break;
+                }
                 float ms = 0.0f;
                 (void)hipEventElapsedTime(&ms, t0, t1);
                 if (ms < 0.97f * best) {
@@ -437,25 +517,29 @@ int autotune(rtm_hip_batch* b) {
         g.tuned = true;
         b->dirty = true;
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    return RTM_HIP_OK;
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    return rc;
 }
 
 int admit_from_queue(rtm_hip_batch* b) {
     while (!b->queue.empty()) {
         Job* j = b->queue.front();
         if (b->cfg.max_active > 0 && (int)b->active.size() >= b->cfg.max_active) break;
-        if (b->used + j->bytes > b->budget) {
+        const size_t need = admission_bytes(*j);
+        if (b->used + need > b->budget) {
             if (b->active.empty())
-                return b->fail(RTM_HIP_ERR_MEMORY, "job %lld needs %.2f GiB, more than the %.2f GiB budget", j->id,
-                               j->bytes / 1073741824.0, b->budget / 1073741824.0);
+                return b->fail(RTM_HIP_ERR_MEMORY,
+                               "job %lld needs %.3f GiB with no job active; %.3f GiB of the %.3f GiB budget "
+                               "are held by resident models", j->id, need / kGiB, b->used / kGiB, b->budget / kGiB);
             break;
         }
         b->queue.pop_front();
         if (int rc = admit_job(b, *j)) {
             job_free(*j);
             j->state = RTM_HIP_JOB_FAILED;
+            j->model->refs -= 1;
+            model_maybe_free(b, j->model);
             return rc;
         }
         b->used += j->bytes;
@@ -471,24 +555,24 @@ int retire_finished(rtm_hip_batch* b) {
     bool any = false;
     for (Job* j : b->active) any |= j->steps_left == 0;
     if (!any) return RTM_HIP_OK;
-    hipError_t e = hipStreamSynchronize(b->stream);
-    if (e != hipSuccess) return b->hip_fail(e, "batch synchronisation");
     std::vector<Job*> keep;
     for (Job* j : b->active) {
         if (j->steps_left > 0) {
             keep.push_back(j);
             continue;
         }
-        if (j->stack_ix0 >= 0 && b->d_stack_i) {
-            e = rtm_hip::launch_stack_add(j->desc.image, j->desc.illum, j->grid.nx, j->grid.nz, b->d_stack_i,
+        const Model& m = *j->model;
+        hipError_t e = hipSuccess;
+        if (j->stack_ix0 >= 0 && b->d_stack_i)
+            e = rtm_hip::launch_stack_add(j->desc.image, j->desc.illum, m.grid.nx, m.grid.nz, b->d_stack_i,
                                           b->d_stack_s, b->stack_nz, j->stack_ix0, j->stack_iz0, b->stream);
-            if (e != hipSuccess) return b->hip_fail(e, "FP64 stack update");
-        }
-        j->image.resize(j->n_int);
-        j->illum.resize(j->n_int);
-        e = hipMemcpyAsync(j->image.data(), j->desc.image, j->n_int * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+        j->image.resize(m.n_int);
+        j->illum.resize(m.n_int);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(j->illum.data(), j->desc.illum, j->n_int * sizeof(float), hipMemcpyDeviceToHost,
+            e = hipMemcpyAsync(j->image.data(), j->desc.image, m.n_int * sizeof(float), hipMemcpyDeviceToHost,
+                               b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(j->illum.data(), j->desc.illum, m.n_int * sizeof(float), hipMemcpyDeviceToHost,
                                b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "image download");
@@ -496,8 +580,10 @@ int retire_finished(rtm_hip_batch* b) {
         b->used -= j->bytes;
         j->state = RTM_HIP_JOB_DONE;
         j->t_done = now_s();
+        j->model->refs -= 1;
+        model_maybe_free(b, j->model);
         b->stats.jobs_done += 1;
-        b->stats.cell_steps += 2.0 * j->nt * (double)j->nxe * (double)j->nze;
+        b->stats.cell_steps += 2.0 * m.nt * (double)m.nxe * (double)m.nze;
         b->stats.busy_latency_s += j->t_done - j->t_submit;
     }
     b->active.swap(keep);
@@ -510,7 +596,7 @@ int run_ticks(rtm_hip_batch* b, int n) {
         if (!b->graph_exec) {
             hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-            const int rc = enqueue_tick(b, 0);
+            const int rc = enqueue_tick(b);
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
@@ -520,7 +606,10 @@ int run_ticks(rtm_hip_batch* b, int n) {
             if (e != hipSuccess) return b->hip_fail(e, "graph capture");
             b->graph = g;
             e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) return b->hip_fail(e, "graph instantiation");
+            if (e != hipSuccess) {
+                b->drop_graph();
+                return b->hip_fail(e, "graph instantiation");
+            }
         }
         for (int i = 0; i < n; ++i) {
             hipError_t e = hipGraphLaunch(b->graph_exec, b->stream);
@@ -528,7 +617,7 @@ int run_ticks(rtm_hip_batch* b, int n) {
         }
     } else {
         for (int i = 0; i < n; ++i)
-            if (int rc = enqueue_tick(b, 0)) return rc;
+            if (int rc = enqueue_tick(b)) return rc;
     }
     for (Job* j : b->active) j->steps_left -= n;
     b->stats.ticks += n;
@@ -540,6 +629,20 @@ int check(const rtm_hip_batch* b) {
         g_batch_null_error = "null rtm_hip_batch handle";
         return RTM_HIP_ERR_ARG;
     }
+    return RTM_HIP_OK;
+}
+
+int enqueue_job(rtm_hip_batch* b, std::unique_ptr<Job> j, long long* id) {
+    if (admission_bytes(*j) > b->budget)
+        return b->fail(RTM_HIP_ERR_MEMORY, "job needs %.3f GiB, more than the %.3f GiB budget",
+                       admission_bytes(*j) / kGiB, b->budget / kGiB);
+    j->id = b->next_job++;
+    j->t_submit = now_s();
+    j->model->refs += 1;
+    b->queue.push_back(j.get());
+    if (id) *id = j->id;
+    b->jobs[j->id] = std::move(j);
+    b->stats.jobs_submitted += 1;
     return RTM_HIP_OK;
 }
 
@@ -570,7 +673,10 @@ int rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rtm_hip_ba
         return RTM_HIP_ERR_ARG;
     }
     size_t free_b = 0, total_b = 0;
-    (void)hipMemGetInfo(&free_b, &total_b);
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        g_batch_null_error = "cannot query device memory";
+        return RTM_HIP_ERR_DEVICE;
+    }
     b->budget = b->cfg.memory_budget ? b->cfg.memory_budget : (size_t)(0.9 * (double)free_b);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         g_batch_null_error = "stream creation failed";
@@ -586,6 +692,8 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     (void)hipStreamSynchronize(b->stream);
     b->drop_graph();
     for (Job* j : b->active) job_free(*j);
+    for (auto& [id, j] : b->jobs) model_release(*j->model);
+    for (auto& [id, m] : b->models) model_release(*m);
     for (auto& [half, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
@@ -599,85 +707,90 @@ const char* rtm_hip_batch_last_error(const rtm_hip_batch* b) {
     return b ? b->err.c_str() : g_batch_null_error.c_str();
 }
 
+int rtm_hip_batch_add_model(rtm_hip_batch* b, const rtm_hip_grid* grid, const float* v, const rtm_hip_params* params,
+                            int nt, float dt, int* model_id) {
+    if (int rc = check(b)) return rc;
+    auto m = std::make_shared<Model>();
+    if (int rc = model_init(b, *m, grid, v, params, nt, dt)) return rc;
+    if (b->used + m->bytes > b->budget)
+        return b->fail(RTM_HIP_ERR_MEMORY, "model needs %.3f GiB; %.3f GiB of the %.3f GiB budget are in use",
+                       m->bytes / kGiB, b->used / kGiB, b->budget / kGiB);
+    (void)hipSetDevice(b->device);
+    if (int rc = model_upload(b, *m)) return rc;
+    m->registered = true;
+    m->id = b->next_model++;
+    b->models[m->id] = m;
+    if (model_id) *model_id = m->id;
+    return RTM_HIP_OK;
+}
+
+int rtm_hip_batch_remove_model(rtm_hip_batch* b, int model_id) {
+    if (int rc = check(b)) return rc;
+    auto it = b->models.find(model_id);
+    if (it == b->models.end()) return b->fail(RTM_HIP_ERR_ARG, "unknown model %d", model_id);
+    it->second->removing = true;
+    model_maybe_free(b, it->second);
+    b->models.erase(it);
+    return RTM_HIP_OK;
+}
+
 int rtm_hip_batch_job_bytes(rtm_hip_batch* b, const rtm_hip_job* job, size_t* bytes) {
     if (int rc = check(b)) return rc;
     if (!job || !bytes) return b->fail(RTM_HIP_ERR_ARG, "null job");
+    auto m = std::make_shared<Model>();
+    if (int rc = model_init(b, *m, &job->grid, job->v, &job->params, job->nt, job->dt)) return rc;
     Job j;
-    j.grid = job->grid;
-    j.params = job->params;
-    j.nt = job->nt;
-    j.dt = job->dt;
-    j.rx.assign(job->shot.rx, job->shot.rx + std::max(job->shot.nrec, 0));
-    j.trace_nt = job->shot.nt;
-    j.traces.resize((size_t)std::max(job->shot.nrec, 0) * std::max(job->shot.nt, 0));
-    if (int rc = derive(b, j)) return rc;
-    *bytes = j.bytes;
+    if (int rc = job_init(b, j, m, &job->shot, -1, -1)) return rc;
+    *bytes = admission_bytes(j);
     return RTM_HIP_OK;
 }
 
 int rtm_hip_batch_submit(rtm_hip_batch* b, const rtm_hip_job* job, long long* id) {
     if (int rc = check(b)) return rc;
-    if (!job || !job->v || !job->shot.rx || !job->shot.rz || !job->shot.traces)
-        return b->fail(RTM_HIP_ERR_ARG, "job needs a velocity model, receivers and traces");
+    if (!job) return b->fail(RTM_HIP_ERR_ARG, "null job");
+    auto m = std::make_shared<Model>();
+    if (int rc = model_init(b, *m, &job->grid, job->v, &job->params, job->nt, job->dt)) return rc;
     auto j = std::make_unique<Job>();
-    j->grid = job->grid;
-    j->params = job->params;
-    j->nt = job->nt;
-    j->dt = job->dt;
-    j->sx = job->shot.sx;
-    j->sz = job->shot.sz;
-    const int nrec = std::max(job->shot.nrec, 0);
-    j->rx.assign(job->shot.rx, job->shot.rx + nrec);
-    j->rz.assign(job->shot.rz, job->shot.rz + nrec);
-    j->trace_nt = job->shot.nt;
-    j->traces.assign(job->shot.traces, job->shot.traces + (size_t)nrec * std::max(job->shot.nt, 0));
-    j->stack_ix0 = job->stack_ix0;
-    j->stack_iz0 = job->stack_iz0;
-    if (int rc = derive(b, *j)) return rc;
-    if (j->stack_ix0 >= 0) {
-        if (!b->d_stack_i) return b->fail(RTM_HIP_ERR_ARG, "job has a stack position but no stack is set");
-        if (j->stack_iz0 < 0 || j->stack_ix0 + j->grid.nx > b->stack_nx || j->stack_iz0 + j->grid.nz > b->stack_nz)
-            return b->fail(RTM_HIP_ERR_ARG, "job image does not fit in the stack at its position");
-    }
-    j->v.assign(job->v, job->v + j->n_int);
-    if (j->bytes > b->budget)
-        return b->fail(RTM_HIP_ERR_MEMORY, "job needs %.2f GiB, more than the %.2f GiB budget", j->bytes / 1073741824.0,
-                       b->budget / 1073741824.0);
-    j->id = b->next_id++;
-    j->t_submit = now_s();
-    b->queue.push_back(j.get());
-    if (id) *id = j->id;
-    b->jobs[j->id] = std::move(j);
-    b->stats.jobs_submitted += 1;
-    return RTM_HIP_OK;
+    if (int rc = job_init(b, *j, m, &job->shot, job->stack_ix0, job->stack_iz0)) return rc;
+    return enqueue_job(b, std::move(j), id);
+}
+
+int rtm_hip_batch_submit_shot(rtm_hip_batch* b, int model_id, const rtm_hip_shot* shot, int stack_ix0,
+                              int stack_iz0, long long* id) {
+    if (int rc = check(b)) return rc;
+    auto it = b->models.find(model_id);
+    if (it == b->models.end()) return b->fail(RTM_HIP_ERR_ARG, "unknown model %d", model_id);
+    auto j = std::make_unique<Job>();
+    if (int rc = job_init(b, *j, it->second, shot, stack_ix0, stack_iz0)) return rc;
+    return enqueue_job(b, std::move(j), id);
 }
 
 int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
     if (int rc = check(b)) return rc;
     (void)hipSetDevice(b->device);
+    const double t0 = now_s();
     long long ran = 0;
+    int rc = RTM_HIP_OK;
     for (;;) {
-        if (int rc = admit_from_queue(b)) return rc;
+        if ((rc = admit_from_queue(b))) break;
         if (b->active.empty()) break;
-        if (b->dirty)
-            if (int rc = rebuild(b)) return rc;
-        if (int rc = autotune(b)) return rc;
-        if (b->dirty)
-            if (int rc = rebuild(b)) return rc;
-        // run until the next job finishes (every job moves one step per tick)
+        if (b->dirty && (rc = rebuild(b))) break;
+        if ((rc = autotune(b))) break;
+        if (b->dirty && (rc = rebuild(b))) break;
+        // every job moves one step per tick: run until the next one finishes
         int n = b->active.front()->steps_left;
         for (Job* j : b->active) n = std::min(n, j->steps_left);
         if (max_ticks > 0) n = (int)std::min<long long>(n, max_ticks - ran);
         if (n <= 0) break;
-        const double t0 = now_s();
-        if (int rc = run_ticks(b, n)) return rc;
+        if ((rc = run_ticks(b, n))) break;
         ran += n;
-        if (int rc = retire_finished(b)) return rc;
-        b->stats.run_seconds += now_s() - t0;
-        if (!until_done && max_ticks > 0 && ran >= max_ticks) break;
-        if (!until_done && max_ticks <= 0) break;
+        if ((rc = retire_finished(b))) break;
+        if (!until_done && (max_ticks <= 0 || ran >= max_ticks)) break;
+        if (until_done && max_ticks > 0 && ran >= max_ticks) break;
     }
     hipError_t e = hipStreamSynchronize(b->stream);
+    b->stats.run_seconds += now_s() - t0;
+    if (rc) return rc;
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "batch run");
 }
 
@@ -685,9 +798,9 @@ int rtm_hip_batch_job_state(rtm_hip_batch* b, long long id, int* state, double* 
     if (int rc = check(b)) return rc;
     auto it = b->jobs.find(id);
     if (it == b->jobs.end()) return b->fail(RTM_HIP_ERR_ARG, "unknown job %lld", id);
-    if (state) *state = it->second->state;
-    if (latency_s)
-        *latency_s = it->second->state == RTM_HIP_JOB_DONE ? it->second->t_done - it->second->t_submit : -1.0;
+    const Job& j = *it->second;
+    if (state) *state = j.state;
+    if (latency_s) *latency_s = j.state == RTM_HIP_JOB_DONE ? j.t_done - j.t_submit : -1.0;
     return RTM_HIP_OK;
 }
 
@@ -706,17 +819,27 @@ int rtm_hip_batch_result(rtm_hip_batch* b, long long id, float* image, float* il
 int rtm_hip_batch_set_stack(rtm_hip_batch* b, int nx, int nz) {
     if (int rc = check(b)) return rc;
     if (nx <= 0 || nz <= 0) return b->fail(RTM_HIP_ERR_ARG, "stack size must be positive");
+    for (Job* j : b->queue)
+        if (j->stack_ix0 >= 0) return b->fail(RTM_HIP_ERR_ARG, "queued jobs use the current stack");
+    for (Job* j : b->active)
+        if (j->stack_ix0 >= 0) return b->fail(RTM_HIP_ERR_ARG, "active jobs use the current stack");
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
     b->d_stack_i = b->d_stack_s = nullptr;
+    b->stack_nx = b->stack_nz = 0;
     const size_t n = (size_t)nx * nz;
     hipError_t e = hipMalloc(&b->d_stack_i, n * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&b->d_stack_s, n * sizeof(double));
     if (e == hipSuccess) e = hipMemset(b->d_stack_i, 0, n * sizeof(double));
     if (e == hipSuccess) e = hipMemset(b->d_stack_s, 0, n * sizeof(double));
-    if (e != hipSuccess) return b->hip_fail(e, "FP64 stack");
+    if (e != hipSuccess) {
+        if (b->d_stack_i) (void)hipFree(b->d_stack_i);
+        if (b->d_stack_s) (void)hipFree(b->d_stack_s);
+        b->d_stack_i = b->d_stack_s = nullptr;
+        return b->hip_fail(e, "FP64 stack");
+    }
     b->stack_nx = nx;
     b->stack_nz = nz;
     return RTM_HIP_OK;
@@ -739,6 +862,7 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
     *out = b->stats;
     out->jobs_active = (int)b->active.size();
     out->jobs_queued = (int)b->queue.size();
+    out->models_resident = (int)b->models.size();
     out->bytes_in_use = b->used;
     out->bytes_budget = b->budget;
     int k = 0;

@@ -202,6 +202,7 @@ class BatchEngine:
             raise RuntimeError(self.lib.rtm_hip_batch_last_error(None).decode())
         self.h = h
         self.sizes: dict[int, int] = {}
+        self.model_sizes: dict[int, int] = {}
 
     def close(self):
         if getattr(self, "h", None):
@@ -235,6 +236,33 @@ class BatchEngine:
         j.shot = _capi.shot_struct(shot, keep)
         j.stack_ix0, j.stack_iz0 = (-1, -1) if stack_at is None else (int(stack_at[0]), int(stack_at[1]))
         return j
+
+    def add_model(self, model: VelocityModel, params: RTMParams, nt: int, dt: float,
+                  snapshot_format: int = SNAP_F32) -> int:
+        """Resident model (the arguments of HipEngine.setup); returns its id."""
+        g = _capi.Grid(model.nx, model.nz, model.dx, model.dz, model.ox, model.oz)
+        p = _capi.Params(params.nb, params.order, params.sponge_alpha, params.store_interval,
+                         params.f0, int(params.verbose), snapshot_format, STACK_F32)
+        v = np.ascontiguousarray(model.v, dtype=np.float32).reshape(-1)
+        mid = C.c_int()
+        self._check(self.lib.rtm_hip_batch_add_model(self.h, C.byref(g), _capi.fptr(v), C.byref(p), int(nt),
+                                                     float(dt), C.byref(mid)))
+        self.model_sizes[mid.value] = model.nx * model.nz
+        return int(mid.value)
+
+    def remove_model(self, model_id: int):
+        self._check(self.lib.rtm_hip_batch_remove_model(self.h, int(model_id)))
+        self.model_sizes.pop(model_id, None)
+
+    def submit_shot(self, model_id: int, shot: ShotRecord, stack_at=None) -> int:
+        """Queue one shot on a resident model."""
+        keep: list = []
+        s = _capi.shot_struct(shot, keep)
+        ix0, iz0 = (-1, -1) if stack_at is None else (int(stack_at[0]), int(stack_at[1]))
+        jid = C.c_longlong()
+        self._check(self.lib.rtm_hip_batch_submit_shot(self.h, int(model_id), C.byref(s), ix0, iz0, C.byref(jid)))
+        self.sizes[jid.value] = self.model_sizes[model_id]
+        return int(jid.value)
 
     def job_bytes(self, model, params, nt, dt, shot, snapshot_format: int = SNAP_F32) -> int:
         keep: list = []

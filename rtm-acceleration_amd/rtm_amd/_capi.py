@@ -71,7 +71,7 @@ class BatchStats(C.Structure):
     _fields_ = [("jobs_submitted", C.c_longlong), ("jobs_done", C.c_longlong),
                 ("jobs_active", C.c_int), ("jobs_queued", C.c_int), ("ticks", C.c_longlong),
                 ("rebuilds", C.c_longlong), ("cell_steps", C.c_double), ("run_seconds", C.c_double),
-                ("busy_latency_s", C.c_double), ("bytes_in_use", C.c_size_t),
+                ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
                 ("group_tile_x", C.c_int * 6), ("group_dpp", C.c_int * 6)]
 
@@ -103,6 +103,10 @@ SIGNATURES = {
     "rtm_hip_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
     "rtm_hip_batch_destroy": (None, [_H]),
     "rtm_hip_batch_last_error": (C.c_char_p, [_H]),
+    "rtm_hip_batch_add_model": (C.c_int, [_H, C.POINTER(Grid), _f32p, C.POINTER(Params), C.c_int, C.c_float,
+                                          C.POINTER(C.c_int)]),
+    "rtm_hip_batch_remove_model": (C.c_int, [_H, C.c_int]),
+    "rtm_hip_batch_submit_shot": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
     "rtm_hip_batch_job_bytes": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_size_t)]),
     "rtm_hip_batch_submit": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_longlong)]),
     "rtm_hip_batch_run": (C.c_int, [_H, C.c_int, C.c_int]),

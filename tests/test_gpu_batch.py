@@ -161,3 +161,49 @@ def test_batch_errors(jobs):
         b.result(jid)
         with pytest.raises(RuntimeError, match="unknown job"):
             b.state(jid)
+
+
+def test_resident_models_serve_many_shots(hip_engine_cls, jobs):
+    """Shots on registered (resident) models: each equals the plain engine's
+    migration of that one shot; models survive their jobs until removed."""
+    import rtm_amd
+    print("case: resident models, several shots each", flush=True)
+    picks = [jobs[0], jobs[2]]
+    with rtm_amd.BatchEngine(0) as b:
+        mids, expect = [], {}
+        for (m, p, nt, dt, s, f) in picks:
+            mids.append(b.add_model(m, p, nt, dt, snapshot_format=f))
+        resident_bytes = b.stats()["bytes_in_use"]
+        assert resident_bytes > 0 and b.stats()["models_resident"] == 2
+        for round_ in range(2):
+            ids = []
+            for k, (m, p, nt, dt, s, f) in enumerate(picks):
+                for sx in (0.2, 0.7):
+                    shot = rtm_amd.ShotRecord(sx * (m.nx - 1) * m.dx, s.sz, s.rx, s.rz, s.nt, s.dt,
+                                              s.traces * (1.0 + round_))
+                    jid = b.submit_shot(mids[k], shot)
+                    ids.append(jid)
+                    if jid not in expect:
+                        with hip_engine_cls() as eng:
+                            eng.setup(m, p, nt, dt, snapshot_format=f)
+                            expect[jid] = eng.migrate([shot])
+            b.run()
+            assert b.stats()["bytes_in_use"] == resident_bytes     # jobs freed, models kept
+            for jid in ids:
+                I, S = b.result(jid)
+                assert bits_equal(expect[jid][0], I), f"job {jid}"
+                assert bits_equal(expect[jid][1], S), f"job {jid}"
+        # removal with a shot still queued: the model lives until that shot is done
+        m, p, nt, dt, s, f = picks[0]
+        jid = b.submit_shot(mids[0], s)
+        b.remove_model(mids[0])
+        b.run()
+        I, _ = b.result(jid)
+        with hip_engine_cls() as eng:
+            eng.setup(m, p, nt, dt, snapshot_format=f)
+            assert bits_equal(eng.migrate([s])[0], I)
+        b.remove_model(mids[1])
+        st = b.stats()
+        assert st["models_resident"] == 0 and st["bytes_in_use"] == 0
+        with pytest.raises(RuntimeError, match="unknown model"):
+            b.submit_shot(mids[0], s)
