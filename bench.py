@@ -117,6 +117,127 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
                       f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
 
 
+def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0):
+    """The batch engine on the same shots: one resident model, every shot
+    submitted at once and advanced together, one launch per time step
+    (hipGraph replay).  Returns throughput and per-shot latency."""
+    import rtm_amd
+    with rtm_amd.BatchEngine(0) as b:
+        mid = b.add_model(model, params, nt, dt, snapshot_format)
+        b.submit_shot(mid, warm_shot)             # warm-up: step-kernel autotune, graph capture
+        b.run()
+        t0 = time.perf_counter()
+        ids = [b.submit_shot(mid, s) for s in shots]
+        b.run()
+        elapsed = time.perf_counter() - t0
+        lat = [b.state(j)[1] for j in ids]
+        st = b.stats()
+        img = sum(b.result(j)[0].astype(np.float64) for j in ids)
+    n_ext = (model.nx + 2 * params.nb) * (model.nz + 2 * params.nb)
+    return {"shots": len(shots), "elapsed_s": round(elapsed, 4),
+            "value": round(2.0 * nt * n_ext * len(shots) / elapsed * 1e-9, 3), "unit": "Gcell-steps/s",
+            "shots_per_s": round(len(shots) / elapsed, 4),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
+            "snapshots": ["f32", "f16", "bf16"][snapshot_format], "groups": st["groups"],
+            "image_checksum": float(np.abs(img).sum())}
+
+
+def crops_workload(args):
+    """64 aperture crops (widths U[400,1200], nt U[2000,8000], orders 8/12)
+    batched, against the plain engine migrating the same crops one by one."""
+    import rtm_amd
+    from rtm_amd import serve
+    specs, reqs = serve.crop_batch(args.crops)
+    print(f"bench: modelling {len(reqs)} crop shots", file=sys.stderr, flush=True)
+    for sp in specs.values():
+        with rtm_amd.HipEngine(0) as e:
+            e.setup(sp.model, sp.params, sp.nt, sp.dt)
+            sp.shots[0].traces = e.model_shot(sp.shots[0])
+    cells = sum(sp.cell_steps for sp in specs.values())
+    print("bench: per-shot loop", file=sys.stderr, flush=True)
+    plain_t = []
+    t_all = time.perf_counter()
+    for k in sorted(specs):
+        sp = specs[k]
+        t = time.perf_counter()
+        with rtm_amd.HipEngine(0) as e:
+            e.setup(sp.model, sp.params, sp.nt, sp.dt)
+            e.set_step_kernel(rtm_amd.STEP_MARCH_DPP, 4)    # no per-crop autotuning
+            e.migrate([sp.shots[0]])
+        plain_t.append(time.perf_counter() - t)
+    plain_elapsed = time.perf_counter() - t_all
+    print("bench: batched", file=sys.stderr, flush=True)
+    with rtm_amd.BatchEngine(0) as b:
+        t0 = time.perf_counter()
+        ids = [b.submit(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
+               for sp in (specs[k] for k in sorted(specs))]
+        b.run()
+        elapsed = time.perf_counter() - t0
+        lat = [b.state(j)[1] for j in ids]
+        st = b.stats()
+    value = cells / elapsed * 1e-9
+    done = np.cumsum(plain_t)                     # all submitted at t=0, served in order
+    return {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded faulted-layer model, aperture crops; traces modelled on the GPU)",
+        "config": {"workload": f"{len(reqs)} aperture crops of the 2301x751 @4 m model, widths U[400,1200], "
+                               "nz 751, nt U[2000,8000], dt 0.35 ms, orders 8/12, nb 60, store 4, FP32 "
+                               "snapshots, all submitted at once to the batch engine", "engine": "hip-batched"},
+        "shots_per_s": round(len(reqs) / elapsed, 4),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
+        "plain_per_shot_loop": {"timing": "setup + migrate per crop, march-dpp tile 4 step kernel",
+                                "value": round(cells / plain_elapsed * 1e-9, 3),
+                                "shots_per_s": round(len(reqs) / plain_elapsed, 4),
+                                "latency_ms": {"p50": round(float(np.percentile(done, 50)) * 1e3, 2),
+                                               "p99": round(float(np.percentile(done, 99)) * 1e3, 2)}},
+        "speedup_vs_plain": round(plain_elapsed / elapsed, 3),
+        "batch": {"ticks": st["ticks"], "rebuilds": st["rebuilds"], "groups": st["groups"]},
+    }
+
+
+def served_workload(args):
+    """Poisson streams over three resident models, swept in rate, against the
+    plain engine serving the same requests serially (FIFO)."""
+    from rtm_amd import serve
+    specs = serve.served_models()
+    print("bench: modelling traces of the served models", file=sys.stderr, flush=True)
+    serve.model_traces(specs)
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = []
+    for rate in rates:
+        reqs = serve.served_stream(specs, rate, args.jobs, seed=int(rate * 1000) + 7)
+        print(f"bench: serving {len(reqs)} requests at {rate}/s", file=sys.stderr, flush=True)
+        batched = serve.serve(specs, reqs)
+        plain = serve.serial_plain(specs, reqs)
+        sweep.append({"rate_per_s": rate, "batched": _round(batched), "plain_serial": _round(plain),
+                      "shots_per_s_ratio": round(batched["shots_per_s"] / plain["shots_per_s"], 3),
+                      "p99_ratio": round(batched["latency_p99_s"] / plain["latency_p99_s"], 3)})
+    top = sweep[-1]
+    return {
+        "metric": METRIC, "value": round(top["batched"]["gcell_steps_per_s"], 3), "unit": "Gcell-steps/s",
+        "n_gpus": 1, "steps": args.jobs, "warmup": 0,
+        "ms_per_step": round(top["batched"]["makespan_s"] / args.jobs * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded models; traces modelled on the GPU; seeded Poisson arrivals)",
+        "config": {"workload": "served stream: requests uniform over 201x151 @10 m (nt 1400), "
+                               "1201x401 @8 m (nt 4000), 2301x751 @4 m (nt 8000) resident models, order 8, "
+                               "FP32 fields, FP16 snapshots; value at the highest rate",
+                   "engine": "hip-batched", "rates": rates, "jobs_per_rate": args.jobs},
+        "shots_per_s": round(top["batched"]["shots_per_s"], 4),
+        "latency_ms": {"p50": round(top["batched"]["latency_p50_s"] * 1e3, 2),
+                       "p99": round(top["batched"]["latency_p99_s"] * 1e3, 2)},
+        "sweep": sweep,
+    }
+
+
+def _round(d):
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +249,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nt", type=int, default=8000, help="time steps per shot (8000 = the workload)")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30, help="copy-probe buffer size")
+    ap.add_argument("--workload", choices=["plain", "crops", "served"], default="plain",
+                    help="plain: the 2301x751 per-shot line (default), with the batch engine on the same "
+                         "shots; crops: 64 aperture crops batched vs one by one; served: Poisson streams")
+    ap.add_argument("--no-batched", action="store_true", help="plain workload without the batched comparison")
+    ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
+    ap.add_argument("--rates", default="2,8,32,64", help="arrival rates (1/s) of the served workload")
+    ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -137,6 +265,10 @@ def main():
     import rtm_amd
     if rtm_amd.device_count() < 1:
         sys.exit("bench: no HIP device")
+    if args.workload != "plain":
+        line = crops_workload(args) if args.workload == "crops" else served_workload(args)
+        print(json.dumps(line), flush=True)
+        return
     K, W = args.steps, args.warmup
     model, params, nt, dt, shots = workload(K + W, args.nt)
     copy_peak = rtm_amd.bandwidth_probe(0, args.probe_bytes, 5)
@@ -201,6 +333,11 @@ def main():
                      "sampled_launches": ks["sampled"]},
         "image_checksum": float(np.abs(img).sum()),
     }
+    if not args.no_batched:
+        print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
+        bt = batched_same_shots(model, params, nt, dt, shots[W:W + K], shots[0])
+        bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
+        line["batched"] = bt
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(model, params, dt, shots[0])

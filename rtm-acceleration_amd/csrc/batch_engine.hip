@@ -119,6 +119,7 @@ struct rtm_hip_batch {
     std::deque<Job*> queue;
     std::vector<Job*> active;
     JobDesc* d_jobs = nullptr;
+    rtm_hip::StepArgs* d_args = nullptr;             // per job, at its clock
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by stencil half-width
     bool dirty = true;
@@ -403,10 +404,13 @@ int rebuild(rtm_hip_batch* b) {
     b->drop_graph();
     if (b->active.size() > b->jobs_cap) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
+        if (b->d_args) (void)hipFree(b->d_args);
+        b->d_jobs = nullptr;
+        b->d_args = nullptr;
         b->jobs_cap = std::max<size_t>(b->active.size(), 2 * b->jobs_cap);
         hipError_t e = hipMalloc(&b->d_jobs, b->jobs_cap * sizeof(JobDesc));
+        if (e == hipSuccess) e = hipMalloc(&b->d_args, b->jobs_cap * sizeof(rtm_hip::StepArgs));
         if (e != hipSuccess) {
-            b->d_jobs = nullptr;
             b->jobs_cap = 0;
             return b->hip_fail(e, "job descriptors");
         }
@@ -450,6 +454,8 @@ int rebuild(rtm_hip_batch* b) {
     }
     if (!descs.empty()) {
         hipError_t e = hipMemcpy(b->d_jobs, descs.data(), descs.size() * sizeof(JobDesc), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)descs.size(), 0, 0, b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "descriptor upload");
     }
     b->dirty = false;
@@ -460,11 +466,11 @@ int rebuild(rtm_hip_batch* b) {
 int enqueue_tick(rtm_hip_batch* b) {
     for (auto& [half, g] : b->groups) {
         if (g.ntiles == 0) continue;
-        hipError_t e = rtm_hip::launch_batch_step(half, g.tile_x, g.dpp, b->d_jobs, g.d_tile_job, g.ntiles, 0,
-                                                  b->stream);
+        hipError_t e = rtm_hip::launch_batch_step(half, g.tile_x, g.dpp, b->d_jobs, b->d_args, g.d_tile_job,
+                                                  g.ntiles, b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
     }
-    hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), b->stream);
+    hipError_t e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 1, 0, b->stream);
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
@@ -491,9 +497,12 @@ int autotune(rtm_hip_batch* b) {
                 g.tile_x = tx;
                 g.dpp = dpp;
                 if ((rc = rebuild(b))) break;
+                e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 0, 1, b->stream);
+                if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                 for (int rep = 0; rep < 6 && !rc; ++rep) {
                     if (rep == 2) (void)hipEventRecord(t0, b->stream);
-                    e = rtm_hip::launch_batch_step(half, tx, dpp, b->d_jobs, g.d_tile_job, g.ntiles, 1, b->stream);
+                    e = rtm_hip::launch_batch_step(half, tx, dpp, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
+                                                   b->stream);
                     if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                 }
                 if (rc) break;
@@ -697,6 +706,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     for (auto& [half, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_args) (void)hipFree(b->d_args);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
     (void)hipStreamDestroy(b->stream);

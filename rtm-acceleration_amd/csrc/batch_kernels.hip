@@ -20,15 +20,19 @@ namespace {
 
 using namespace device;
 
+__device__ __forceinline__ float* rotate(const JobDesc& J, int r) {
+    return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
+}
+
 // Step arguments of job J at its current clock.
 __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
     StepArgs a{};
     const int fwd = J.phase == 0;
     const int s = J.step;
     const int base = fwd ? 0 : J.bwd_base;
-    a.p_prev = J.fld[(base + s) % 3];
-    a.p_cur = J.fld[(base + s + 1) % 3];
-    a.p_next = J.fld[(base + s + 2) % 3];
+    a.p_prev = rotate(J, (base + s) % 3);
+    a.p_cur = rotate(J, (base + s + 1) % 3);
+    a.p_next = rotate(J, (base + s + 2) % 3);
     a.zero_prev = s <= 1;
     a.zero_cur = s == 0;
     a.vdt2 = J.vdt2;
@@ -64,40 +68,50 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
         }
     } else if (stored) {
         a.snap16_max = J.snap_max + slot;
-        if (s == 0) a.pending_field = J.fld[J.bwd_base];   // last forward field
+        if (s == 0) a.pending_field = rotate(J, J.bwd_base);   // last forward field
         else a.snap16_in = J.snap16 + slot * J.n_int;
     }
     return a;
 }
 
+// One tick of every job of half-width H.  The job and tile of a wave are
+// wave-uniform (read-first-lane), so the job's step arguments, prepared by
+// k_clock, are read with scalar loads as a kernel argument would be.
 template <int H, int TX, bool DPP>
 __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
-                                                         const int* __restrict__ tile_job, int ntiles, int dry) {
-    const int wave = (int)threadIdx.x / kTileZ;
+                                                         const StepArgs* __restrict__ args,
+                                                         const int* __restrict__ tile_job, int ntiles) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (ntiles + kWaves - 1) / kWaves;
     const int g = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (g >= ntiles) return;
-    const JobDesc& J = jobs[tile_job[g]];
-    if (J.phase > 1) return;                 // finished job: its tiles idle
-    const StepArgs a = job_args(J, dry);
-    const int t = g - J.tile_first;
-    if (J.phase == 0) march_tile<H, TX, Dir::Forward, DPP>(a, t, lane);
-    else march_tile<H, TX, Dir::Backward, DPP>(a, t, lane);
+    const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
+    const int phase = jobs[jid].phase;
+    if (phase > 1) return;                   // finished job: its tiles idle
+    const int t = g - jobs[jid].tile_first;
+    if (phase == 0) march_tile<H, TX, Dir::Forward, DPP>(args[jid], t, lane);
+    else march_tile<H, TX, Dir::Backward, DPP>(args[jid], t, lane);
 }
 
-__global__ __launch_bounds__(64) void k_advance(JobDesc* jobs, int njobs) {
+// Job clocks (advance != 0: one step; at nt, the next phase) and the step
+// arguments of every job at its clock.
+__global__ __launch_bounds__(64) void k_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry) {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j >= njobs) return;
     JobDesc& J = jobs[j];
     if (J.phase > 1) return;
-    const int s = J.step + 1;
-    if (s == J.nt) {
-        J.step = 0;
-        J.phase = J.phase + 1;
-    } else {
-        J.step = s;
+    if (advance) {
+        const int s = J.step + 1;
+        if (s == J.nt) {
+            J.step = 0;
+            J.phase = J.phase + 1;
+        } else {
+            J.step = s;
+        }
+        if (J.phase > 1) return;
     }
+    args[j] = job_args(J, dry);
 }
 
 __global__ __launch_bounds__(256) void k_stack_add(const float* image, const float* illum, int nx, int nz,
@@ -111,14 +125,14 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 }
 
 template <int H>
-hipError_t launch_h(int tile_x, bool dpp, const JobDesc* jobs, const int* tile_job, int ntiles, int dry,
-                    hipStream_t s) {
+hipError_t launch_h(int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args, const int* tile_job,
+                    int ntiles, hipStream_t s) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
-#define RTM_BATCH(TX)                                                                                     \
-    do {                                                                                                  \
-        if (dpp) hipLaunchKernelGGL((k_batch<H, TX, true>), grid, block, 0, s, jobs, tile_job, ntiles, dry); \
-        else hipLaunchKernelGGL((k_batch<H, TX, false>), grid, block, 0, s, jobs, tile_job, ntiles, dry);  \
+#define RTM_BATCH(TX)                                                                                        \
+    do {                                                                                                     \
+        if (dpp) hipLaunchKernelGGL((k_batch<H, TX, true>), grid, block, 0, s, jobs, args, tile_job, ntiles); \
+        else hipLaunchKernelGGL((k_batch<H, TX, false>), grid, block, 0, s, jobs, args, tile_job, ntiles);  \
     } while (0)
     switch (tile_x) {
         case 4: RTM_BATCH(4); break;
@@ -132,16 +146,16 @@ hipError_t launch_h(int tile_x, bool dpp, const JobDesc* jobs, const int* tile_j
 
 }  // namespace
 
-hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int dry, hipStream_t stream) {
+hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args,
+                             const int* tile_job, int ntiles, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
-        case 2: return launch_h<2>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
-        case 3: return launch_h<3>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
-        case 4: return launch_h<4>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
-        case 5: return launch_h<5>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
-        case 6: return launch_h<6>(tile_x, dpp, jobs, tile_job, ntiles, dry, stream);
+        case 1: return launch_h<1>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 2: return launch_h<2>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 3: return launch_h<3>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 4: return launch_h<4>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 5: return launch_h<5>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 6: return launch_h<6>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -154,9 +168,10 @@ hipError_t launch_stack_add(const float* image, const float* illum, int nx, int 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_advance(JobDesc* jobs, int njobs, hipStream_t stream) {
+hipError_t launch_batch_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry, hipStream_t stream) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_advance, dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, stream, jobs, njobs);
+    hipLaunchKernelGGL(k_clock, dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, stream, jobs, args, njobs,
+                       advance, dry);
     return hipGetLastError();
 }
 

@@ -4,7 +4,8 @@
 // interval and tables).  The device holds one descriptor per job; a tick
 // advances every active job by one step of its current phase (forward, then
 // backward with imaging) with one launch per stencil half-width present, and
-// a one-thread-per-job kernel then moves the job clocks.  The launch
+// a one-thread-per-job kernel then moves the job clocks and prepares every
+// job's step arguments for the next tick.  The launch
 // arguments never change between ticks, so a tick can be captured once in a
 // hipGraph and replayed.
 #pragma once
@@ -44,17 +45,21 @@ struct JobDesc {
 };
 
 // One tick for the jobs of one half-width H: tile_job[g] names the job of
-// global tile g; DPP as in the per-shot march kernel.  dry != 0 computes the
-// step without touching images (autotuning on live state).
-hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int dry, hipStream_t stream);
+// global tile g, args[j] holds job j's step arguments at its clock (written
+// by launch_batch_clock); DPP as in the per-shot march kernel.
+hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args,
+                             const int* tile_job, int ntiles, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.
 hipError_t launch_stack_add(const float* image, const float* illum, int nx, int nz, double* stack_i,
                             double* stack_s, int stack_nz, int ix0, int iz0, hipStream_t stream);
 
-// Job clocks: step += 1; at nt, next phase.
-hipError_t launch_batch_advance(JobDesc* jobs, int njobs, hipStream_t stream);
+// Job clocks and step arguments: advance != 0 moves every clock one step
+// (at nt, to the next phase) first; then args[j] = the step arguments of job
+// j at its clock, with the dry flag given (dry steps write only p^{n+1}: no
+// image, illumination or snapshot update; used to time candidate kernels on
+// live state).
+hipError_t launch_batch_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry, hipStream_t stream);
 
 }  // namespace rtm_hip

@@ -9,6 +9,18 @@
 namespace rtm_hip {
 namespace device {
 
+// Every buffer the step touches is device global memory.  Pointers that reach
+// a kernel through memory (the batched kernel reads its arguments from a
+// per-job table) would otherwise be accessed with flat instructions, which
+// also count against the LDS/scalar wait counter; the views below make every
+// access a global one, as for pointers passed as kernel arguments.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ global_t<T>* gbl(T* p) {
+    return (global_t<T>*)p;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Give XCD k the
 // k-th contiguous block of logical tiles so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
@@ -53,13 +65,14 @@ template <Dir D>
 __device__ __forceinline__ float apply_entries(const StepArgs& a, int cell, float out, float v2, int e0,
                                                int e1) {
     for (int e = e0; e < e1; ++e) {
-        const Entry en = a.entries[e];
-        if (en.cell != cell) continue;
+        const int en_cell = gbl(a.entries)[e].cell;
+        const int en_code = gbl(a.entries)[e].code;
+        if (en_cell != cell) continue;
         if constexpr (D == Dir::Forward) {
-            if (en.code < 0) out = out + v2 * a.wavelet[a.it];
-            else a.rec_out[(ptrdiff_t)en.code * a.rec_stride + a.it] = out;
+            if (en_code < 0) out = out + v2 * gbl(a.wavelet)[a.it];
+            else gbl(a.rec_out)[(ptrdiff_t)en_code * a.rec_stride + a.it] = out;
         } else {
-            out = out + v2 * a.traces[(ptrdiff_t)en.code * a.trace_stride + a.it];
+            out = out + v2 * gbl(a.traces)[(ptrdiff_t)en_code * a.trace_stride + a.it];
         }
     }
     return out;
@@ -83,8 +96,8 @@ __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
     c.store32 = d == Dir::Forward && wide && a.snap_out != nullptr;
     c.store_max = d == Dir::Forward && !wide && a.snap_max != nullptr;
     c.quantise = d == Dir::Forward && !wide && a.quant_out != nullptr;
-    c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*a.snap16_max) : 0;
-    c.k_q = c.quantise ? snapshot_scale_exponent(*a.quant_max) : 0;
+    c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*gbl(a.snap16_max)) : 0;
+    c.k_q = c.quantise ? snapshot_scale_exponent(*gbl(a.quant_max)) : 0;
     return c;
 }
 
@@ -92,8 +105,9 @@ __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
 // a decoded 16-bit code, or a 16-bit snapshot still pending in its field
 // (encoded and decoded here, which gives the value the store would hold).
 __device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx& c, ptrdiff_t j, ptrdiff_t i) {
-    if (a.snap_fmt == (int)SnapFmt::F32) return a.snap_in[j];
-    const uint16_t code = a.pending_field ? encode_snapshot(a.pending_field[i], c.k_in, a.snap_fmt) : a.snap16_in[j];
+    if (a.snap_fmt == (int)SnapFmt::F32) return gbl(a.snap_in)[j];
+    const uint16_t code =
+        a.pending_field ? encode_snapshot(gbl(a.pending_field)[i], c.k_in, a.snap_fmt) : gbl(a.snap16_in)[j];
     return decode_snapshot(code, c.k_in, (SnapFmt)a.snap_fmt);
 }
 
@@ -121,7 +135,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
 #pragma unroll
     for (int j = 0; j < NQ; ++j) {
         const int x = x0 - H + j;
-        q[j] = (z_load && x < a.nxe && !a.zero_cur) ? a.p_cur[(ptrdiff_t)x * a.ldz + z] : 0.0f;
+        q[j] = (z_load && x < a.nxe && !a.zero_cur) ? gbl(a.p_cur)[(ptrdiff_t)x * a.ldz + z] : 0.0f;
     }
     float pv[TX], vv[TX], wxv[TX];
     constexpr int NI = D == Dir::Backward ? TX : 1;   // imaging operands
@@ -131,20 +145,20 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
         const int x = x0 + r;
         const bool ok = z_out && x < a.nxe - H;
         const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
-        pv[r] = (ok && !a.zero_prev) ? a.p_prev[i] : 0.0f;
-        vv[r] = ok ? a.vdt2[i] : 0.0f;
-        wxv[r] = x < a.nxe ? a.wx[x] : 0.0f;
+        pv[r] = (ok && !a.zero_prev) ? gbl(a.p_prev)[i] : 0.0f;
+        vv[r] = ok ? gbl(a.vdt2)[i] : 0.0f;
+        wxv[r] = x < a.nxe ? gbl(a.wx)[x] : 0.0f;
         if constexpr (D == Dir::Backward) {
             const int jx = x - a.nb;
             const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
             fv[r] = in ? snapshot_value(a, sc, j, i) : 0.0f;
-            iv[r] = in ? a.image[j] : 0.0f;
-            sv[r] = in ? a.illum[j] : 0.0f;
+            iv[r] = in ? gbl(a.image)[j] : 0.0f;
+            sv[r] = in ? gbl(a.illum)[j] : 0.0f;
         }
     }
-    const float wzv = z_load ? a.wz[z] : 0.0f;
-    const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+    const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
+    const int e0 = gbl(a.tile_ptr)[t], e1 = gbl(a.tile_ptr)[t + 1];
     float vmax = 0.0f;
 
 #pragma unroll
@@ -174,19 +188,19 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
         float out = (2.0f * pc - prev) + vv[r] * lap;
         out = out * w;
         out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
-        a.p_next[i] = out;
+        gbl(a.p_next)[i] = out;
 
         const int jx = x - a.nb;
         if (z_int && jx >= 0 && jx < a.nx) {
             const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
             if constexpr (D == Dir::Forward) {
-                if (sc.store32 && !a.dry) a.snap_out[j] = out;
+                if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
                 if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-                if (sc.quantise && !a.dry) a.quant_out[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
+                if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
             } else {
                 if (sc.imaging && !a.dry) {
-                    a.image[j] = iv[r] + fv[r] * out;
-                    a.illum[j] = sv[r] + fv[r] * fv[r];
+                    gbl(a.image)[j] = iv[r] + fv[r] * out;
+                    gbl(a.illum)[j] = sv[r] + fv[r] * fv[r];
                 }
             }
         }
