@@ -16,8 +16,9 @@ Reported (one JSON line):
   value        Gcell-steps/s, cell-steps = 2 nt (nx+2nb)(nz+2nb) per shot
                (the reference's accounting, src/benchmark/benchmark.cpp:22-25)
   roofline     the backward step kernel (the dominant one): algorithmic bytes
-               per launch / its mean duration from HIP events recorded on the
-               engine's stream around sampled launches in the timed region
+               per launch / its mean duration, from HIP events recorded on the
+               engine's stream around each shot's 8000 back-to-back backward
+               step launches in the timed region (divided by 8000)
   cpu_baseline the reference's cpu-opt engine, compiled from its own sources
                (oracle/_ref), timed on this host on a bounded sample of the
                same workload (one shot, nt capped to a ~20 s budget)
@@ -245,7 +246,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--family", type=int, default=-1, help="step-kernel family (-1 built-in, 0 LDS tile, 1 march)")
     ap.add_argument("--tile", type=int, default=0, help="step-kernel tile width (0 = built-in)")
-    ap.add_argument("--sample-every", type=int, default=32, help="event-time every N-th step launch")
+    ap.add_argument("--sample-every", type=int, default=-1,
+                    help="kernel timing: -1 = events around each shot's whole step loop (default), "
+                         "N > 0 = events around every N-th step launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nt", type=int, default=8000, help="time steps per shot (8000 = the workload)")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30, help="copy-probe buffer size")
@@ -315,7 +318,7 @@ def main():
         "config": {"workload": "2301x751 @4 m, nt 8000, dt 0.35 ms, order 8, nb 60, store 4, "
                                "767 receivers, plain per-shot engine, FP32 snapshots",
                    "engine": "hip",
-                   "step_kernel": {d: {"family": ["lds-tile", "march", "march-dpp"][info["step_family"][k]],
+                   "step_kernel": {d: {"family": ["lds-tile", "march", "march-dpp", "stream"][info["step_family"][k]],
                                        "tile_x": info["tile_x"][k], "autotuned": bool(info["autotuned"][k])}
                                    for k, d in enumerate(("forward", "backward"))},
                    "ldz": info["ldz"]},

@@ -39,6 +39,7 @@ extern "C" {
 #define RTM_HIP_STEP_LDS_TILE  0
 #define RTM_HIP_STEP_MARCH     1
 #define RTM_HIP_STEP_MARCH_DPP 2
+#define RTM_HIP_STEP_STREAM    3
 
 /* Snapshot storage formats (rtm_hip_setup's params->snapshot_format). */
 #define RTM_HIP_SNAP_F32   0      /* the reference's format (src/rtm/forward.cpp:83-88) */
@@ -176,7 +177,11 @@ int         rtm_hip_model_shot(rtm_hip_engine* h, const rtm_hip_shot* shot, floa
 int         rtm_hip_get_times(const rtm_hip_engine* h, rtm_hip_times* out);
 int         rtm_hip_reset_times(rtm_hip_engine* h);
 
-/* Step-kernel timing for the roofline figure; every = 0 turns sampling off.
+/* Step-kernel timing for the roofline figure.  every > 0: HIP events around
+ * every `every`-th step launch (each such launch also carries the events'
+ * end-of-kernel overhead); every < 0: one event pair around all step
+ * launches of each forward/backward loop, counted as nt launches (the mean
+ * then includes the short gaps between back-to-back launches); 0: off.
  * No reference counterpart (the reference never filled times.kernel,
  * include/rtm_engine.hpp:11). */
 int         rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every);
@@ -184,9 +189,11 @@ int         rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_sta
 
 /* Step-kernel mapping: family RTM_HIP_STEP_MARCH / RTM_HIP_STEP_MARCH_DPP (one
  * wave64 per depth strip, all loads up front; depth neighbours by ds_bpermute
- * or by DPP lane shifts) or RTM_HIP_STEP_LDS_TILE (LDS-staged 64-cell tiles),
- * or -1 for the built-in choice; tile_x = output columns per tile (4, 8 or 16),
- * 0 for the built-in choice.  The built-in choice is an autotuner that times
+ * or by DPP lane shifts), RTM_HIP_STEP_STREAM (the strip marches along its
+ * columns with p^n in a sliding register window) or RTM_HIP_STEP_LDS_TILE
+ * (LDS-staged 64-cell tiles), or -1 for the built-in choice; tile_x = output
+ * columns per tile (4, 8 or 16; 8, 16 or 32 for the stream kernel), 0 for the
+ * built-in choice.  The built-in choice is an autotuner that times
  * every family and width on the engine's grid before the first propagation
  * of each direction after setup.  Results are identical for every choice;
  * takes effect at the next forward/backward.  No reference counterpart (the CUDA
@@ -228,6 +235,9 @@ typedef struct {
     size_t memory_budget;        /* device bytes for jobs; 0 = 90% of free memory    */
     int    max_active;           /* cap on jobs on the device at once; 0 = no cap    */
     int    use_graph;            /* 1: capture one tick in a hipGraph and replay it  */
+    int    kernel_family;        /* 0: autotuned per stencil order; else RTM_HIP_STEP_MARCH,
+                                    RTM_HIP_STEP_MARCH_DPP or RTM_HIP_STEP_STREAM      */
+    int    tile_x;               /* with kernel_family: its tile width (0: default)  */
 } rtm_hip_batch_config;
 
 /* One shot migration: the arguments of setup (grid, velocity, params, nt, dt)
@@ -257,7 +267,7 @@ typedef struct {
     int       models_resident;   /* registered models (rtm_hip_batch_add_model)     */
     size_t    bytes_in_use, bytes_budget;
     int       ngroups;           /* stencil half-widths with a launch per tick     */
-    int       group_half[6], group_tile_x[6], group_dpp[6];
+    int       group_half[6], group_tile_x[6], group_family[6];   /* RTM_HIP_STEP_* */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

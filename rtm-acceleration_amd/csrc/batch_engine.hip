@@ -96,8 +96,8 @@ struct Job {
 };
 
 struct Group {
+    rtm_hip::Family family = rtm_hip::Family::kMarchDpp;
     int tile_x = 4;
-    bool dpp = true;
     bool tuned = false;
     int ntiles = 0;
     int* d_tile_job = nullptr;
@@ -466,7 +466,7 @@ int rebuild(rtm_hip_batch* b) {
 int enqueue_tick(rtm_hip_batch* b) {
     for (auto& [half, g] : b->groups) {
         if (g.ntiles == 0) continue;
-        hipError_t e = rtm_hip::launch_batch_step(half, g.tile_x, g.dpp, b->d_jobs, b->d_args, g.d_tile_job,
+        hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, b->d_args, g.d_tile_job,
                                                   g.ntiles, b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
     }
@@ -474,11 +474,21 @@ int enqueue_tick(rtm_hip_batch* b) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Tile width and cross-lane method per half-width, picked by timing dry
+// Kernel family and tile width per half-width, picked by timing dry
 // steps (no clock advance, no image or snapshot update: the next real step
 // rewrites the same p^{n+1}) on the live jobs; a candidate must beat the
 // best by 3 % to replace it.
 int autotune(rtm_hip_batch* b) {
+    if (b->cfg.kernel_family != 0) {
+        for (auto& [half, g] : b->groups) {
+            if (g.tuned) continue;
+            g.family = static_cast<rtm_hip::Family>(b->cfg.kernel_family);
+            g.tile_x = b->cfg.tile_x ? b->cfg.tile_x : (g.family == rtm_hip::Family::kStream ? 16 : 8);
+            g.tuned = true;
+            b->dirty = true;
+        }
+        return RTM_HIP_OK;
+    }
     bool any = false;
     for (auto& [half, g] : b->groups) any |= g.ntiles > 0 && !g.tuned;
     if (!any) return RTM_HIP_OK;
@@ -490,18 +500,18 @@ int autotune(rtm_hip_batch* b) {
         if (rc || g.ntiles == 0 || g.tuned) continue;
         float best = 1e30f;
         int best_tx = g.tile_x;
-        bool best_dpp = g.dpp;
-        for (int tx : {4, 8, 16}) {
-            for (bool dpp : {true, false}) {
+        rtm_hip::Family best_fam = g.family;
+        for (auto fam : {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch}) {
+            for (int tx : rtm_hip::tile_widths(fam)) {
                 if (rc) break;
                 g.tile_x = tx;
-                g.dpp = dpp;
+                g.family = fam;
                 if ((rc = rebuild(b))) break;
                 e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 0, 1, b->stream);
                 if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
-                for (int rep = 0; rep < 6 && !rc; ++rep) {
+                for (int rep = 0; rep < 10 && !rc; ++rep) {
                     if (rep == 2) (void)hipEventRecord(t0, b->stream);
-                    e = rtm_hip::launch_batch_step(half, tx, dpp, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
+                    e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
                                                    b->stream);
                     if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                 }
@@ -517,12 +527,12 @@ int autotune(rtm_hip_batch* b) {
                 if (ms < 0.97f * best) {
                     best = ms;
                     best_tx = tx;
-                    best_dpp = dpp;
+                    best_fam = fam;
                 }
             }
         }
         g.tile_x = best_tx;
-        g.dpp = best_dpp;
+        g.family = best_fam;
         g.tuned = true;
         b->dirty = true;
     }
@@ -681,6 +691,17 @@ int rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rtm_hip_ba
         g_batch_null_error = "max_active must be >= 0";
         return RTM_HIP_ERR_ARG;
     }
+    const int fam = b->cfg.kernel_family, tx = b->cfg.tile_x;
+    const bool fam_ok = fam == 0 || fam == RTM_HIP_STEP_MARCH || fam == RTM_HIP_STEP_MARCH_DPP ||
+                        fam == RTM_HIP_STEP_STREAM;
+    bool tx_ok = tx == 0;
+    if (fam != 0)
+        for (int t : rtm_hip::tile_widths(static_cast<rtm_hip::Family>(fam))) tx_ok |= t == tx;
+    if (!fam_ok || !tx_ok) {
+        g_batch_null_error = "batch kernel family must be 0 (autotuned), march, march-dpp or stream, "
+                             "with a tile width that family has";
+        return RTM_HIP_ERR_ARG;
+    }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         g_batch_null_error = "cannot query device memory";
@@ -785,11 +806,23 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         if ((rc = admit_from_queue(b))) break;
         if (b->active.empty()) break;
         if (b->dirty && (rc = rebuild(b))) break;
-        if ((rc = autotune(b))) break;
-        if (b->dirty && (rc = rebuild(b))) break;
         // every job moves one step per tick: run until the next one finishes
         int n = b->active.front()->steps_left;
         for (Job* j : b->active) n = std::min(n, j->steps_left);
+        // The autotuner times dry steps on the live jobs; a job's first two
+        // steps read p^n and p^{n-1} as zero and skip those loads, so a new
+        // half-width group first runs two steps with the default kernel.
+        int warm = 0;
+        if (b->cfg.kernel_family == 0)
+            for (Job* j : b->active)
+                if (!b->groups[j->model->half].tuned)
+                    warm = std::max(warm, 2 - (2 * j->model->nt - j->steps_left));
+        if (warm > 0) {
+            n = std::min(n, warm);
+        } else {
+            if ((rc = autotune(b))) break;
+            if (b->dirty && (rc = rebuild(b))) break;
+        }
         if (max_ticks > 0) n = (int)std::min<long long>(n, max_ticks - ran);
         if (n <= 0) break;
         if ((rc = run_ticks(b, n))) break;
@@ -880,7 +913,7 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
         if (k >= 6) break;
         out->group_half[k] = half;
         out->group_tile_x[k] = g.tile_x;
-        out->group_dpp[k] = g.dpp ? 1 : 0;
+        out->group_family[k] = (int)g.family;
         ++k;
     }
     out->ngroups = k;

@@ -77,7 +77,7 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
 // One tick of every job of half-width H.  The job and tile of a wave are
 // wave-uniform (read-first-lane), so the job's step arguments, prepared by
 // k_clock, are read with scalar loads as a kernel argument would be.
-template <int H, int TX, bool DPP>
+template <int H, int TX, Family F>
 __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
                                                          const StepArgs* __restrict__ args,
                                                          const int* __restrict__ tile_job, int ntiles) {
@@ -90,8 +90,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restri
     const int phase = jobs[jid].phase;
     if (phase > 1) return;                   // finished job: its tiles idle
     const int t = g - jobs[jid].tile_first;
-    if (phase == 0) march_tile<H, TX, Dir::Forward, DPP>(args[jid], t, lane);
-    else march_tile<H, TX, Dir::Backward, DPP>(args[jid], t, lane);
+    if constexpr (F == Family::kStream) {
+        if (phase == 0) stream_tile<H, TX, Dir::Forward>(args[jid], t, lane);
+        else stream_tile<H, TX, Dir::Backward>(args[jid], t, lane);
+    } else {
+        constexpr bool DPP = F == Family::kMarchDpp;
+        if (phase == 0) march_tile<H, TX, Dir::Forward, DPP>(args[jid], t, lane);
+        else march_tile<H, TX, Dir::Backward, DPP>(args[jid], t, lane);
+    }
 }
 
 // Job clocks (advance != 0: one step; at nt, the next phase) and the step
@@ -125,20 +131,36 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 }
 
 template <int H>
-hipError_t launch_h(int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args, const int* tile_job,
+hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const StepArgs* args, const int* tile_job,
                     int ntiles, hipStream_t s) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
-#define RTM_BATCH(TX)                                                                                        \
-    do {                                                                                                     \
-        if (dpp) hipLaunchKernelGGL((k_batch<H, TX, true>), grid, block, 0, s, jobs, args, tile_job, ntiles); \
-        else hipLaunchKernelGGL((k_batch<H, TX, false>), grid, block, 0, s, jobs, args, tile_job, ntiles);  \
-    } while (0)
-    switch (tile_x) {
-        case 4: RTM_BATCH(4); break;
-        case 8: RTM_BATCH(8); break;
-        case 16: RTM_BATCH(16); break;
-        default: return hipErrorInvalidValue;
+#define RTM_BATCH(TX, F) hipLaunchKernelGGL((k_batch<H, TX, F>), grid, block, 0, s, jobs, args, tile_job, ntiles)
+    if (family == Family::kStream) {
+        switch (tile_x) {
+            case 8: RTM_BATCH(8, Family::kStream); break;
+            case 16: RTM_BATCH(16, Family::kStream); break;
+            case 32: RTM_BATCH(32, Family::kStream); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (family == Family::kMarchDpp) {
+        switch (tile_x) {
+            case 4: RTM_BATCH(4, Family::kMarchDpp); break;
+            case 5: RTM_BATCH(5, Family::kMarchDpp); break;
+            case 6: RTM_BATCH(6, Family::kMarchDpp); break;
+            case 8: RTM_BATCH(8, Family::kMarchDpp); break;
+            case 16: RTM_BATCH(16, Family::kMarchDpp); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (family == Family::kMarch) {
+        switch (tile_x) {
+            case 4: RTM_BATCH(4, Family::kMarch); break;
+            case 8: RTM_BATCH(8, Family::kMarch); break;
+            case 16: RTM_BATCH(16, Family::kMarch); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
     }
 #undef RTM_BATCH
     return hipGetLastError();
@@ -146,16 +168,16 @@ hipError_t launch_h(int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* a
 
 }  // namespace
 
-hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args,
+hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const StepArgs* args,
                              const int* tile_job, int ntiles, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
-        case 2: return launch_h<2>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
-        case 3: return launch_h<3>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
-        case 4: return launch_h<4>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
-        case 5: return launch_h<5>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
-        case 6: return launch_h<6>(tile_x, dpp, jobs, args, tile_job, ntiles, stream);
+        case 1: return launch_h<1>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 2: return launch_h<2>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 3: return launch_h<3>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 4: return launch_h<4>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 5: return launch_h<5>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 6: return launch_h<6>(family, tile_x, jobs, args, tile_job, ntiles, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -46,8 +46,9 @@ struct JobDesc {
 
 // One tick for the jobs of one half-width H: tile_job[g] names the job of
 // global tile g, args[j] holds job j's step arguments at its clock (written
-// by launch_batch_clock); DPP as in the per-shot march kernel.
-hipError_t launch_batch_step(int half, int tile_x, bool dpp, const JobDesc* jobs, const StepArgs* args,
+// by launch_batch_clock); family kMarch, kMarchDpp or kStream with the tile
+// widths of the per-shot kernels.
+hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const StepArgs* args,
                              const int* tile_job, int ntiles, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]

@@ -95,7 +95,12 @@ struct rtm_hip_engine {
     long long sampled[2] = {0, 0};
     double sampled_ms[2] = {0, 0};
     std::vector<EventPair> ev_pool;
-    std::vector<std::pair<int, EventPair>> ev_pending;
+    struct PendingEvents {
+        int kind;            // 0 forward, 1 backward
+        EventPair ev;
+        long long launches;  // step launches between the two events
+    };
+    std::vector<PendingEvents> ev_pending;
     EventPair loop_ev;
 
     ~rtm_hip_engine() { release(); }
@@ -164,8 +169,8 @@ struct rtm_hip_engine {
         }
         ev_pool.clear();
         for (auto& pe : ev_pending) {
-            (void)hipEventDestroy(pe.second.start);
-            (void)hipEventDestroy(pe.second.stop);
+            (void)hipEventDestroy(pe.ev.start);
+            (void)hipEventDestroy(pe.ev.stop);
         }
         ev_pending.clear();
         if (loop_ev.start) (void)hipEventDestroy(loop_ev.start);
@@ -181,6 +186,7 @@ struct rtm_hip_engine {
         if (family_user < 0 && tile_x_user == 0 && tuned[k]) return tuned_geom[k];
         const auto fam = family_user < 0 ? rtm_hip::Family::kMarchDpp : static_cast<rtm_hip::Family>(family_user);
         int tx = tile_x_user;
+        if (tx == 0 && fam == rtm_hip::Family::kStream) tx = 16;
         if (tx == 0) tx = (long long)nxe * ldz >= 4LL * 1024 * 1024 ? 8 : 4;
         return rtm_hip::step_geometry(fam, half, tx, nxe, nze, ldz);
     }
@@ -276,11 +282,11 @@ EventPair take_event(rtm_hip_engine* h) {
 void collect_events(rtm_hip_engine* h) {
     for (auto& pe : h->ev_pending) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, pe.second.start, pe.second.stop) == hipSuccess) {
-            h->sampled_ms[pe.first] += ms;
-            h->sampled[pe.first] += 1;
+        if (hipEventElapsedTime(&ms, pe.ev.start, pe.ev.stop) == hipSuccess) {
+            h->sampled_ms[pe.kind] += ms;
+            h->sampled[pe.kind] += pe.launches;
         }
-        h->ev_pool.push_back(pe.second);
+        h->ev_pool.push_back(pe.ev);
     }
     h->ev_pending.clear();
 }
@@ -379,6 +385,12 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     }
     (void)hipEventRecord(h->loop_ev.start, h->stream);
 
+    // every < 0: one event pair around all step launches of the loop
+    EventPair phase_ev;
+    if (h->timing_every < 0) {
+        phase_ev = take_event(h);
+        (void)hipEventRecord(phase_ev.start, h->stream);
+    }
     int ia = 0, ib = 1, ic = 2;   // p_prev, p_cur, p_next
     for (int step = 0; step < h->nt; ++step) {
         const int it = dir == Dir::Forward ? step : h->nt - 1 - step;
@@ -393,13 +405,17 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         if (sample) ev = take_event(h);
         hipError_t e = rtm_hip::launch_step(dir, h->half, g, a, h->stream, ev.start, ev.stop);
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
-        if (sample) h->ev_pending.push_back({k, ev});
+        if (sample) h->ev_pending.push_back({k, ev, 1});
         h->launches[k] += 1;
         // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
         const int old = ia;
         ia = ib;
         ib = ic;
         ic = old;
+    }
+    if (phase_ev.start) {
+        (void)hipEventRecord(phase_ev.stop, h->stream);
+        h->ev_pending.push_back({k, phase_ev, h->nt});
     }
     // a 16-bit snapshot of the last step has no following step to quantise it
     if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && ((h->nt - 1) % h->store) == 0) {
@@ -432,13 +448,18 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
 int autotune(rtm_hip_engine* h, Dir dir) {
     const int k = dir == Dir::Forward ? 0 : 1;
     if (h->tuned[k] || h->family_user >= 0 || h->tile_x_user != 0) return RTM_HIP_OK;
-    const rtm_hip::Family fams[] = {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kMarch,
-                                    rtm_hip::Family::kLdsTile};
-    const int tiles[] = {4, 8, 16};
+    struct Candidate {
+        rtm_hip::Family family;
+        int tile_x;
+    };
+    std::vector<Candidate> cands;
+    for (auto f : {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch,
+                   rtm_hip::Family::kLdsTile})
+        for (int t : rtm_hip::tile_widths(f)) cands.push_back({f, t});
     int max_tiles = 0;
-    for (auto f : fams)
-        for (int t : tiles)
-            max_tiles = std::max(max_tiles, rtm_hip::step_geometry(f, h->half, t, h->nxe, h->nze, h->ldz).ntiles);
+    for (const Candidate& c : cands)
+        max_tiles = std::max(max_tiles,
+                             rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz).ntiles);
     int* d_ptr = nullptr;
     float* d_img = nullptr;
     if (hipMalloc(&d_ptr, (size_t)(max_tiles + 1) * sizeof(int)) != hipSuccess ||
@@ -459,9 +480,10 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     rtm_hip::StepGeometry best = h->geometry(k);
     int rc = RTM_HIP_OK;
     constexpr int kWarm = 4, kTimed = 32;
-    for (auto f : fams) {
-        for (int t : tiles) {
-            const rtm_hip::StepGeometry g = rtm_hip::step_geometry(f, h->half, t, h->nxe, h->nze, h->ldz);
+    for (const Candidate& c : cands) {
+        {
+            const rtm_hip::StepGeometry g =
+                rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
             a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
             for (int n = 0; n < kWarm + kTimed && !rc; ++n) {
                 if (n == kWarm) (void)hipEventRecord(ev.start, h->stream);
@@ -875,7 +897,6 @@ int rtm_hip_reset_times(rtm_hip_engine* h) {
 
 int rtm_hip_set_kernel_timing(rtm_hip_engine* h, int every) {
     if (int rc = check_handle(h, false)) return rc;
-    if (every < 0) return h->fail(RTM_HIP_ERR_ARG, "sampling interval must be >= 0");
     h->timing_every = every;
     return RTM_HIP_OK;
 }
@@ -893,11 +914,21 @@ int rtm_hip_get_kernel_stats(const rtm_hip_engine* h, rtm_hip_kernel_stats* out)
 
 int rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x) {
     if (int rc = check_handle(h, false)) return rc;
-    if (family < -1 || family > RTM_HIP_STEP_MARCH_DPP)
+    if (family < -1 || family > RTM_HIP_STEP_STREAM)
         return h->fail(RTM_HIP_ERR_ARG, "step kernel family must be -1 or %d..%d", RTM_HIP_STEP_LDS_TILE,
-                       RTM_HIP_STEP_MARCH_DPP);
-    if (tile_x != 0 && tile_x != 4 && tile_x != 8 && tile_x != 16)
-        return h->fail(RTM_HIP_ERR_ARG, "tile width must be 0, 4, 8 or 16");
+                       RTM_HIP_STEP_STREAM);
+    if (tile_x != 0) {
+        bool ok = false;
+        if (family < 0) {
+            ok = tile_x == 4 || tile_x == 8 || tile_x == 16;
+        } else {
+            for (int t : rtm_hip::tile_widths(static_cast<rtm_hip::Family>(family))) ok |= t == tile_x;
+        }
+        if (!ok)
+            return h->fail(RTM_HIP_ERR_ARG, "tile width %d is not available for step kernel family %d "
+                           "(march-dpp: 4, 5, 6, 8, 16; march, lds-tile: 4, 8, 16; stream: 8, 16, 32)",
+                           tile_x, family);
+    }
     h->family_user = family;
     h->tile_x_user = tile_x;
     return RTM_HIP_OK;

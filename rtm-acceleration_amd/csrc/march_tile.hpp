@@ -112,9 +112,113 @@ __device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx
 }
 
 // ---------------------------------------------------------------------------
-// March mapping: one wave64 = one tile of (64 - 2H) x TX outputs.  DPP picks
-// how depth neighbours cross lanes: chained one-lane DPP shifts, or
+// Pieces shared by the march and stream mappings.
+// ---------------------------------------------------------------------------
+
+// Laplacian at the lane's cell of column c of the p^n window q (q[c-H..c+H]
+// hold columns x-H..x+H): c0 p + sum_k (cx_k (x+k + x-k) + cz_k (z+k + z-k)),
+// summed in the reference's order (src/rtm/forward.cpp:38-44).  Depth
+// neighbours come from other lanes: chained one-lane DPP shifts, or
 // ds_bpermute (__shfl_up/__shfl_down) by k lanes.
+template <int H, bool DPP>
+__device__ __forceinline__ float laplacian(const StepArgs& a, const float* q, int c) {
+    const float pc = q[c];
+    float lap = a.coef.c0 * pc;
+    float zp = pc, zm = pc;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        if constexpr (DPP) {
+            zp = from_lane_above(zp);                           // p^n at z + k
+            zm = from_lane_below(zm);                           // p^n at z - k
+        } else {
+            zp = __shfl_down(pc, (unsigned)k, kTileZ);
+            zm = __shfl_up(pc, (unsigned)k, kTileZ);
+        }
+        const float tx_k = a.coef.cx[k] * (q[c + k] + q[c - k]);
+        const float tz_k = a.coef.cz[k] * (zp + zm);
+        lap = lap + (tx_k + tz_k);
+    }
+    return lap;
+}
+
+// Per-column operands of one output column x: p^{n-1}, (v dt)^2, the x
+// taper and, for a backward imaging step, the snapshot value and the image
+// and illumination being accumulated.
+struct ColumnOps {
+    float pv, vv, wx, fv, iv, sv;
+};
+
+template <int H, Dir D>
+__device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_out,
+                                                bool z_int, int jz) {
+    ColumnOps o;
+    const bool ok = z_out && x < a.nxe - H;
+    const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
+    o.pv = (ok && !a.zero_prev) ? gbl(a.p_prev)[i] : 0.0f;
+    o.vv = ok ? gbl(a.vdt2)[i] : 0.0f;
+    o.wx = x < a.nxe ? gbl(a.wx)[x] : 0.0f;
+    o.fv = o.iv = o.sv = 0.0f;
+    if constexpr (D == Dir::Backward) {
+        const int jx = x - a.nb;
+        const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
+        const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+        o.fv = in ? snapshot_value(a, sc, j, i) : 0.0f;
+        o.iv = in ? gbl(a.image)[j] : 0.0f;
+        o.sv = in ? gbl(a.illum)[j] : 0.0f;
+    }
+    return o;
+}
+
+__device__ __forceinline__ float load_cur(const StepArgs& a, int x, int z, bool z_load) {
+    return (z_load && x < a.nxe && !a.zero_cur) ? gbl(a.p_cur)[(ptrdiff_t)x * a.ldz + z] : 0.0f;
+}
+
+// Leapfrog update, damping, injection and the snapshot / imaging epilogue of
+// one output cell (x, z) (src/rtm/forward.cpp:45, :56-63, :112-118;
+// src/rtm/backward.cpp:57-75):
+//   p^{n+1} = w ((2 p^n - w p^{n-1}) + (v dt)^2 lap), w = wx[x] wz[z]
+// (p^{n-1} is damped on read: the reference damps it in place after the
+// previous step, which gives the same value; injection cells lie where
+// w == 1, so injecting after the damping multiply is the reference's
+// inject-then-damp).  Entries [e0, e1) may hit this cell.
+template <Dir D>
+__device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
+                                            float pc, float lap, const ColumnOps& o, float wzv, int e0, int e1,
+                                            float& vmax) {
+    const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
+    const float w = o.wx * wzv;
+    const float prev = o.pv * w;
+    float out = (2.0f * pc - prev) + o.vv * lap;
+    out = out * w;
+    out = apply_entries<D>(a, (int)i, out, o.vv, e0, e1);
+    gbl(a.p_next)[i] = out;
+
+    const int jx = x - a.nb;
+    if (z_int && jx >= 0 && jx < a.nx) {
+        const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
+        if constexpr (D == Dir::Forward) {
+            if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
+            if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
+            if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
+        } else {
+            if (sc.imaging && !a.dry) {
+                gbl(a.image)[j] = o.iv + o.fv * out;
+                gbl(a.illum)[j] = o.sv + o.fv * o.fv;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc, float vmax, int lane) {
+    if (sc.store_max && !a.dry) {
+        vmax = wave_max(vmax);
+        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
+// of the tile issued before any use.
 // ---------------------------------------------------------------------------
 template <int H, int TX, Dir D, bool DPP>
 __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
@@ -130,85 +234,97 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
     const SnapCtx sc = snap_ctx(a, D);
 
-    // ---- every load of the tile, issued before any use ------------------
     float q[NQ];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) {
-        const int x = x0 - H + j;
-        q[j] = (z_load && x < a.nxe && !a.zero_cur) ? gbl(a.p_cur)[(ptrdiff_t)x * a.ldz + z] : 0.0f;
-    }
-    float pv[TX], vv[TX], wxv[TX];
-    constexpr int NI = D == Dir::Backward ? TX : 1;   // imaging operands
-    float fv[NI], iv[NI], sv[NI];
+    for (int j = 0; j < NQ; ++j) q[j] = load_cur(a, x0 - H + j, z, z_load);
+    ColumnOps ops[TX];
 #pragma unroll
-    for (int r = 0; r < TX; ++r) {
-        const int x = x0 + r;
-        const bool ok = z_out && x < a.nxe - H;
-        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
-        pv[r] = (ok && !a.zero_prev) ? gbl(a.p_prev)[i] : 0.0f;
-        vv[r] = ok ? gbl(a.vdt2)[i] : 0.0f;
-        wxv[r] = x < a.nxe ? gbl(a.wx)[x] : 0.0f;
-        if constexpr (D == Dir::Backward) {
-            const int jx = x - a.nb;
-            const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
-            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-            fv[r] = in ? snapshot_value(a, sc, j, i) : 0.0f;
-            iv[r] = in ? gbl(a.image)[j] : 0.0f;
-            sv[r] = in ? gbl(a.illum)[j] : 0.0f;
-        }
-    }
+    for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
     const int e0 = gbl(a.tile_ptr)[t], e1 = gbl(a.tile_ptr)[t + 1];
     float vmax = 0.0f;
 
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
-        const float pc = q[r + H];
-        float lap = a.coef.c0 * pc;
-        float zp = pc, zm = pc;
-#pragma unroll
-        for (int k = 1; k <= H; ++k) {
-            if constexpr (DPP) {
-                zp = from_lane_above(zp);                           // p^n at z + k
-                zm = from_lane_below(zm);                           // p^n at z - k
-            } else {
-                zp = __shfl_down(pc, (unsigned)k, kTileZ);
-                zm = __shfl_up(pc, (unsigned)k, kTileZ);
-            }
-            const float tx_k = a.coef.cx[k] * (q[r + H + k] + q[r + H - k]);
-            const float tz_k = a.coef.cz[k] * (zp + zm);
-            lap = lap + (tx_k + tz_k);
-        }
+        const float lap = laplacian<H, DPP>(a, q, r + H);
         const int x = x0 + r;
         if (!z_out || x >= a.nxe - H) continue;
+        finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, e0, e1, vmax);
+    }
+    finish_tile(a, sc, vmax, lane);
+}
 
-        const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
-        const float w = wxv[r] * wzv;
-        const float prev = pv[r] * w;
-        float out = (2.0f * pc - prev) + vv[r] * lap;
-        out = out * w;
-        out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
-        gbl(a.p_next)[i] = out;
+// ---------------------------------------------------------------------------
+// Stream mapping: one wave64 = one strip of (64 - 2H) depth cells marching
+// along XS columns in chunks of C.  p^n sits in a sliding register window of
+// C + 2H columns, so each column is loaded once per strip (the march mapping
+// loads TX + 2H columns for TX outputs); the loads of chunk k+1 are issued
+// before chunk k is computed and stored.  Injection entries of the strip are
+// sorted by cell (column-major), so a uniform cursor hands each column its
+// own entries.
+// ---------------------------------------------------------------------------
+template <int H, int XS, Dir D>
+__device__ __forceinline__ void stream_tile(const StepArgs& a, int t, int lane) {
+    constexpr int C = 4;                     // columns per chunk
+    constexpr int NCH = XS / C;
+    constexpr int W = C + 2 * H;             // p^n window of one chunk
+    constexpr int TZ = kTileZ - 2 * H;
+    static_assert(XS % C == 0, "strip width must be a multiple of the chunk");
 
-        const int jx = x - a.nb;
-        if (z_int && jx >= 0 && jx < a.nx) {
-            const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-            if constexpr (D == Dir::Forward) {
-                if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
-                if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-                if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
-            } else {
-                if (sc.imaging && !a.dry) {
-                    gbl(a.image)[j] = iv[r] + fv[r] * out;
-                    gbl(a.illum)[j] = sv[r] + fv[r] * fv[r];
-                }
-            }
+    const int tz = t % a.ntz, tx = t / a.ntz;
+    const int z = tz * TZ + lane;
+    const int x0 = H + tx * XS;
+    const bool z_load = z < a.nze;
+    const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
+    const int jz = z - a.nb;
+    const bool z_int = z_out && jz >= 0 && jz < a.nz;
+    const SnapCtx sc = snap_ctx(a, D);
+
+    float q[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) q[j] = load_cur(a, x0 - H + j, z, z_load);
+    ColumnOps ops[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) ops[r] = column_ops<H, D>(a, sc, x0 + r, z, z_out, z_int, jz);
+    const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
+    int ec = gbl(a.tile_ptr)[t];
+    const int e1 = gbl(a.tile_ptr)[t + 1];
+    float vmax = 0.0f;
+
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        const int xc = x0 + k * C;
+        // next chunk's loads, in flight while this chunk computes
+        float nq[C];
+        ColumnOps nops[C];
+        if (k + 1 < NCH) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) nq[j] = load_cur(a, xc + C + H + j, z, z_load);
+#pragma unroll
+            for (int r = 0; r < C; ++r) nops[r] = column_ops<H, D>(a, sc, xc + C + r, z, z_out, z_int, jz);
+        }
+#pragma unroll
+        for (int r = 0; r < C; ++r) {
+            const int x = xc + r;
+            // entries of column x: cells [x ldz, (x+1) ldz)
+            int ee = ec;
+            const long long col_end = (long long)(x + 1) * a.ldz;
+            while (ee < e1 && gbl(a.entries)[ee].cell < col_end) ++ee;
+            const float lap = laplacian<H, true>(a, q, r + H);
+            if (z_out && x < a.nxe - H)
+                finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, ec, ee, vmax);
+            ec = ee;
+        }
+        if (k + 1 < NCH) {
+#pragma unroll
+            for (int j = 0; j < 2 * H; ++j) q[j] = q[j + C];
+#pragma unroll
+            for (int j = 0; j < C; ++j) q[2 * H + j] = nq[j];
+#pragma unroll
+            for (int r = 0; r < C; ++r) ops[r] = nops[r];
         }
     }
-    if (sc.store_max && !a.dry) {
-        vmax = wave_max(vmax);
-        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
-    }
+    finish_tile(a, sc, vmax, lane);
 }
 
 }  // namespace device

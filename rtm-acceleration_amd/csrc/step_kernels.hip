@@ -31,7 +31,7 @@
 // 2^k the power of two that keeps |code| < 2^15.  Scaling by 2^k is exact, so
 // the only error is the FP16/BF16 rounding; decoding is code * 2^-k, exact.
 //
-// Two mappings (Family in step_kernels.hpp) share the arithmetic above.
+// Three mappings (Family in step_kernels.hpp) share the arithmetic above.
 //
 // kMarch / kMarchDpp (default).  Each wave64 owns one tile: a depth strip of
 // 64 consecutive cells, of which the middle 64 - 2H lanes are outputs (strips
@@ -44,6 +44,11 @@
 // step is bound by the bytes in flight, not by instruction issue, so this
 // layout trades a 2H/64 depth overlap and a 2H/tile_x column overlap (both
 // re-read from L2) for one memory round trip per wave.
+//
+// kStream.  The march strip, marching along tile_x (8, 16 or 32) columns in
+// chunks of 4 with p^n in a sliding register window of 4 + 2H columns: each
+// column of p^n is loaded once per strip instead of (tile_x + 2H)/tile_x
+// times, and the next chunk's loads are in flight while a chunk computes.
 //
 // kLdsTile.  A 256-thread workgroup (4 x wave64) owns 64 depth cells x tile_x
 // columns; the tile of p^n plus its halo is staged in LDS with 256-byte row
@@ -183,6 +188,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     march_tile<H, TX, D, DPP>(a, t, lane);
 }
 
+// Stream mapping: one wave per strip of XS columns (march_tile.hpp).
+template <int H, int XS, Dir D>
+__global__ __launch_bounds__(kBlockThreads) void k_stream(StepArgs a) {
+    const int wave = (int)threadIdx.x / kTileZ;
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
+    const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
+    if (t >= a.ntiles) return;               // whole wave leaves together
+    stream_tile<H, XS, D>(a, t, lane);
+}
+
 // Quantise the interior of an extended field into a 16-bit snapshot.
 __global__ __launch_bounds__(256) void k_quantise(const float* p, int ldz, int nb, int nx, int nz, int fmt,
                                                   const float* maxv, uint16_t* out) {
@@ -223,6 +239,13 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
             case 16: RTM_LAUNCH((k_step<H, 4, D>)); break;
             default: return hipErrorInvalidValue;
         }
+    } else if (g.family == Family::kStream) {
+        switch (g.tile_x) {
+            case 8:  RTM_LAUNCH((k_stream<H, 8, D>)); break;
+            case 16: RTM_LAUNCH((k_stream<H, 16, D>)); break;
+            case 32: RTM_LAUNCH((k_stream<H, 32, D>)); break;
+            default: return hipErrorInvalidValue;
+        }
     } else if (g.family == Family::kMarch) {
         switch (g.tile_x) {
             case 4:  RTM_LAUNCH((k_march<H, 4, D, false>)); break;
@@ -233,6 +256,8 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
     } else {
         switch (g.tile_x) {
             case 4:  RTM_LAUNCH((k_march<H, 4, D, true>)); break;
+            case 5:  RTM_LAUNCH((k_march<H, 5, D, true>)); break;
+            case 6:  RTM_LAUNCH((k_march<H, 6, D, true>)); break;
             case 8:  RTM_LAUNCH((k_march<H, 8, D, true>)); break;
             case 16: RTM_LAUNCH((k_march<H, 16, D, true>)); break;
             default: return hipErrorInvalidValue;

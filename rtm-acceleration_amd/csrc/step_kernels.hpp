@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -92,7 +94,9 @@ struct StepArgs {
 //             it loads every operand of its tile up front (no LDS, no
 //             barrier) and takes depth neighbours from other lanes through
 //             ds_bpermute (kMarch) or chained DPP lane shifts (kMarchDpp).
-enum class Family : int { kLdsTile = 0, kMarch = 1, kMarchDpp = 2 };
+//   kStream:  the march strip marching along tile_x columns with p^n in a
+//             sliding register window (DPP depth neighbours).
+enum class Family : int { kLdsTile = 0, kMarch = 1, kMarchDpp = 2, kStream = 3 };
 
 // Tile grid of one launch.  Tiles are numbered x-major over (ntz x ntx):
 // tile = tx * ntz + tz; the injection table is indexed the same way.
@@ -107,6 +111,17 @@ struct StepGeometry {
 };
 
 StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz);
+
+// Tile widths each family is instantiated for.  The odd widths of the DPP
+// march let the tile count of a grid fall just under a multiple of the
+// chip's resident-wave capacity instead of just over it.
+inline std::vector<int> tile_widths(Family f) {
+    switch (f) {
+        case Family::kMarchDpp: return {4, 5, 6, 8, 16};
+        case Family::kStream: return {8, 16, 32};
+        default: return {4, 8, 16};
+    }
+}
 
 inline int tile_of_cell(const StepGeometry& g, int ixe, int ize) {
     return ((ixe - g.x_origin) / g.tile_x) * g.ntz + (ize - g.z_origin) / g.tile_z;

@@ -14,11 +14,11 @@ import numpy as np
 
 from . import _capi
 from ._capi import (JOB_DONE, JOB_FAILED, JOB_QUEUED, JOB_RUNNING, SNAP_BF16, SNAP_F16, SNAP_F32,
-                    STACK_F32, STACK_F64, STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP)
+                    STACK_F32, STACK_F64, STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP, STEP_STREAM)
 from .types import RTMParams, ShotRecord, TimeAxis, VelocityModel
 
 __all__ = ["HipEngine", "BatchEngine", "JOB_QUEUED", "JOB_RUNNING", "JOB_DONE", "JOB_FAILED", "RTMParams", "ShotRecord", "TimeAxis", "VelocityModel",
-           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STACK_F32", "STACK_F64", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "device_count", "mute_direct_wave", "bandwidth_probe",
+           "SNAP_F32", "SNAP_F16", "SNAP_BF16", "STACK_F32", "STACK_F64", "STEP_LDS_TILE", "STEP_MARCH", "STEP_MARCH_DPP", "STEP_STREAM", "device_count", "mute_direct_wave", "bandwidth_probe",
            "library_path"]
 
 
@@ -193,9 +193,10 @@ class BatchEngine:
     each on its own grid, advanced together one time step per tick."""
 
     def __init__(self, device: int = 0, memory_budget: int = 0, max_active: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, kernel_family: int = 0, tile_x: int = 0):
         self.lib = _capi.load()
-        cfg = _capi.BatchConfig(int(memory_budget), int(max_active), int(bool(use_graph)))
+        cfg = _capi.BatchConfig(int(memory_budget), int(max_active), int(bool(use_graph)), int(kernel_family),
+                                int(tile_x))
         h = C.c_void_p()
         rc = self.lib.rtm_hip_batch_create(device, C.byref(cfg), C.byref(h))
         if rc != _capi.OK:
@@ -315,8 +316,9 @@ class BatchEngine:
         self._check(self.lib.rtm_hip_batch_get_stats(self.h, C.byref(s)))
         out = {n: getattr(s, n) for n, _ in _capi.BatchStats._fields_}
         k = s.ngroups
-        out["groups"] = [{"half": s.group_half[i], "tile_x": s.group_tile_x[i], "dpp": s.group_dpp[i]}
-                         for i in range(k)]
-        for n in ("group_half", "group_tile_x", "group_dpp"):
+        names = ["lds-tile", "march", "march-dpp", "stream"]
+        out["groups"] = [{"half": s.group_half[i], "tile_x": s.group_tile_x[i],
+                          "family": names[s.group_family[i]]} for i in range(k)]
+        for n in ("group_half", "group_tile_x", "group_family"):
             del out[n]
         return out

@@ -15,7 +15,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "librtm_hip.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
-STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP = 0, 1, 2
+STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP, STEP_STREAM = 0, 1, 2, 3
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 STACK_F32, STACK_F64 = 0, 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
@@ -59,7 +59,8 @@ class KernelStats(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("memory_budget", C.c_size_t), ("max_active", C.c_int), ("use_graph", C.c_int)]
+    _fields_ = [("memory_budget", C.c_size_t), ("max_active", C.c_int), ("use_graph", C.c_int),
+                ("kernel_family", C.c_int), ("tile_x", C.c_int)]
 
 
 class Job(C.Structure):
@@ -73,7 +74,7 @@ class BatchStats(C.Structure):
                 ("rebuilds", C.c_longlong), ("cell_steps", C.c_double), ("run_seconds", C.c_double),
                 ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
-                ("group_tile_x", C.c_int * 6), ("group_dpp", C.c_int * 6)]
+                ("group_tile_x", C.c_int * 6), ("group_family", C.c_int * 6)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

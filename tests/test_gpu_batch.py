@@ -93,6 +93,21 @@ def test_batch_equals_alone(jobs, alone, use_graph, max_active):
             assert bits_equal(alone[k][1], S), f"job {k}: illumination differs"
 
 
+@pytest.mark.parametrize("family,tile", [(1, 4), (2, 5), (2, 16), (3, 8), (3, 32)])
+def test_batch_kernel_families(jobs, alone, family, tile):
+    """Every batched kernel family and width gives the plain engine's bits."""
+    import rtm_amd
+    print(f"case: batch with kernel family {family} tile {tile}", flush=True)
+    with rtm_amd.BatchEngine(0, kernel_family=family, tile_x=tile) as b:
+        ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+        b.run()
+        assert all(g["tile_x"] == tile for g in b.stats()["groups"])
+        for k, jid in enumerate(ids):
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}"
+            assert bits_equal(alone[k][1], S), f"job {k}"
+
+
 def test_interleaved_submission_and_budget(jobs, alone):
     """Jobs submitted while others run, under a budget that holds two jobs."""
     import rtm_amd
