@@ -135,7 +135,9 @@ def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=
         st = b.stats()
         img = sum(b.result(j)[0].astype(np.float64) for j in ids)
     n_ext = (model.nx + 2 * params.nb) * (model.nz + 2 * params.nb)
+    bytes_shot = algorithmic_bytes(model, params, nt, 4 if snapshot_format == 0 else 2)[1]
     return {"shots": len(shots), "elapsed_s": round(elapsed, 4),
+            "whole_shot_hbm_frac": round(bytes_shot * len(shots) / elapsed * 1e-9 / HBM_PEAK_GBS, 4),
             "value": round(2.0 * nt * n_ext * len(shots) / elapsed * 1e-9, 3), "unit": "Gcell-steps/s",
             "shots_per_s": round(len(shots) / elapsed, 4),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),

@@ -24,8 +24,9 @@
 // Launches.  Jobs are grouped by stencil half-width; each group has one
 // launch per tick over the concatenation of its jobs' tiles, with the tile
 // width and cross-lane method picked once per half-width by timing dry runs
-// on the live jobs.  A tick (the group launches plus the clock kernel) is
-// captured once in a hipGraph and replayed until the job set changes.
+// on the live jobs.  A tick (the group launches plus the clock kernel), and a
+// run of 32 ticks, are captured in hipGraphs and replayed until the job set
+// changes, so one host call advances every job 32 steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,8 +124,9 @@ struct rtm_hip_batch {
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by stencil half-width
     bool dirty = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // captured ticks: graph[0] = one tick, graph[1] = kGraphTicks ticks
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     int stack_nx = 0, stack_nz = 0;
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
@@ -144,10 +146,12 @@ struct rtm_hip_batch {
                     hipGetErrorString(e));
     }
     void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+        for (int k = 0; k < 2; ++k) {
+            if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            graph_exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
     }
 };
 
@@ -400,7 +404,10 @@ int admit_job(rtm_hip_batch* b, Job& j) {
 // ---- launch tables, ticks, autotuning ------------------------------------
 
 // The launch tables of every half-width group and the device descriptors.
+// Ticks still running on the stream read the descriptors, the step arguments
+// and the tables rewritten here, so the stream is drained first.
 int rebuild(rtm_hip_batch* b) {
+    if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess) return b->hip_fail(e, "batch synchronisation");
     b->drop_graph();
     if (b->active.size() > b->jobs_cap) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
@@ -477,7 +484,7 @@ int enqueue_tick(rtm_hip_batch* b) {
 // Kernel family and tile width per half-width, picked by timing dry
 // steps (no clock advance, no image or snapshot update: the next real step
 // rewrites the same p^{n+1}) on the live jobs; a candidate must beat the
-// best by 3 % to replace it.
+// best by 5 % to replace it.
 int autotune(rtm_hip_batch* b) {
     if (b->cfg.kernel_family != 0) {
         for (auto& [half, g] : b->groups) {
@@ -509,22 +516,28 @@ int autotune(rtm_hip_batch* b) {
                 if ((rc = rebuild(b))) break;
                 e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 0, 1, b->stream);
                 if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
-                for (int rep = 0; rep < 10 && !rc; ++rep) {
-                    if (rep == 2) (void)hipEventRecord(t0, b->stream);
-                    e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
-                                                   b->stream);
-                    if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
+                // 2 warm steps, then the best of 3 rounds of 4 timed steps
+                float ms = 1e30f;
+                for (int round = 0; round < 3 && !rc; ++round) {
+                    for (int rep = 0; rep < (round == 0 ? 6 : 4) && !rc; ++rep) {
+                        if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
+                        e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
+                                                       b->stream);
+                        if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
+                    }
+                    if (rc) break;
+                    (void)hipEventRecord(t1, b->stream);
+                    e = hipEventSynchronize(t1);
+                    if (e != hipSuccess) {
+                        rc = b->hip_fail(e, "autotune");
+                        break;
+                    }
+                    float round_ms = 0.0f;
+                    (void)hipEventElapsedTime(&round_ms, t0, t1);
+                    ms = std::min(ms, round_ms);
                 }
                 if (rc) break;
-                (void)hipEventRecord(t1, b->stream);
-                e = hipEventSynchronize(t1);
-                if (e != hipSuccess) {
-                    rc = b->hip_fail(e, "autotune");
-                    break;
-                }
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, t0, t1);
-                if (ms < 0.97f * best) {
+                if (ms < 0.95f * best) {
                     best = ms;
                     best_tx = tx;
                     best_fam = fam;
@@ -610,28 +623,48 @@ int retire_finished(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
+// Captures `ticks` consecutive ticks into graph slot k.
+int capture_ticks(rtm_hip_batch* b, int k, int ticks) {
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return b->hip_fail(e, "graph capture");
+    int rc = RTM_HIP_OK;
+    for (int i = 0; i < ticks && !rc; ++i) rc = enqueue_tick(b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return b->hip_fail(e, "graph capture");
+    b->graph[k] = g;
+    e = hipGraphInstantiate(&b->graph_exec[k], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        b->drop_graph();
+        return b->hip_fail(e, "graph instantiation");
+    }
+    return RTM_HIP_OK;
+}
+
+// Ticks between job-set changes replay captured graphs: kGraphTicks ticks per
+// launch while that many remain (one host call for many small ticks), then
+// single ticks.
+constexpr int kGraphTicks = 32;
+
 int run_ticks(rtm_hip_batch* b, int n) {
     if (b->cfg.use_graph) {
-        if (!b->graph_exec) {
-            hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-            const int rc = enqueue_tick(b);
-            hipGraph_t g = nullptr;
-            e = hipStreamEndCapture(b->stream, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-            b->graph = g;
-            e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                b->drop_graph();
-                return b->hip_fail(e, "graph instantiation");
+        int left = n;
+        if (left >= kGraphTicks) {
+            if (!b->graph_exec[1])
+                if (int rc = capture_ticks(b, 1, kGraphTicks)) return rc;
+            for (; left >= kGraphTicks; left -= kGraphTicks) {
+                hipError_t e = hipGraphLaunch(b->graph_exec[1], b->stream);
+                if (e != hipSuccess) return b->hip_fail(e, "graph launch");
             }
         }
-        for (int i = 0; i < n; ++i) {
-            hipError_t e = hipGraphLaunch(b->graph_exec, b->stream);
+        if (left > 0 && !b->graph_exec[0])
+            if (int rc = capture_ticks(b, 0, 1)) return rc;
+        for (; left > 0; --left) {
+            hipError_t e = hipGraphLaunch(b->graph_exec[0], b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "graph launch");
         }
     } else {

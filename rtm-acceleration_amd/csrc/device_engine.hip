@@ -441,7 +441,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
 }
 
 // Autotuner: times every step-kernel family and tile width on this grid for
-// one direction (32 launches each, no injection entries, imaging or snapshot
+// one direction (best of 3 x 16 launches each, no injection entries, imaging or snapshot
 // traffic on every store-th launch as in the real loop) and keeps the
 // fastest.  Runs once per setup and direction, before the direction's first
 // propagation; the wavefields are reset afterwards by the propagation itself.
@@ -479,14 +479,18 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     float best_ms = 1e30f;
     rtm_hip::StepGeometry best = h->geometry(k);
     int rc = RTM_HIP_OK;
-    constexpr int kWarm = 4, kTimed = 32;
+    // per candidate: 4 warm launches, then the best of 3 rounds of 16 timed
+    // launches (the minimum is the least noisy estimate of a kernel's time)
+    constexpr int kWarm = 4, kRounds = 3, kTimed = 16;
     for (const Candidate& c : cands) {
-        {
-            const rtm_hip::StepGeometry g =
-                rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
-            a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
-            for (int n = 0; n < kWarm + kTimed && !rc; ++n) {
-                if (n == kWarm) (void)hipEventRecord(ev.start, h->stream);
+        const rtm_hip::StepGeometry g = rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
+        a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
+        float cand_ms = 1e30f;
+        int n = 0;
+        for (int round = 0; round < kRounds && !rc; ++round) {
+            const int count = (round == 0 ? kWarm : 0) + kTimed;
+            for (int i = 0; i < count && !rc; ++i, ++n) {
+                if (i == count - kTimed) (void)hipEventRecord(ev.start, h->stream);
                 a.p_prev = h->d_fld[n % 3];
                 a.p_cur = h->d_fld[(n + 1) % 3];
                 a.p_next = h->d_fld[(n + 2) % 3];
@@ -504,14 +508,15 @@ int autotune(rtm_hip_engine* h, Dir dir) {
             }
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, ev.start, ev.stop);
-            // candidates are tried in order of preference; a later one must
-            // win by 3 % so that timing noise does not flip the choice
-            if (ms < 0.97f * best_ms) {
-                best_ms = ms;
-                best = g;
-            }
+            cand_ms = std::min(cand_ms, ms);
         }
         if (rc) break;
+        // candidates are tried in order of preference; a later one must win
+        // by 5 % so that timing noise does not flip the choice
+        if (cand_ms < 0.95f * best_ms) {
+            best_ms = cand_ms;
+            best = g;
+        }
     }
     h->ev_pool.push_back(ev);
     (void)hipStreamSynchronize(h->stream);

@@ -222,3 +222,18 @@ def test_resident_models_serve_many_shots(hip_engine_cls, jobs):
         assert st["models_resident"] == 0 and st["bytes_in_use"] == 0
         with pytest.raises(RuntimeError, match="unknown model"):
             b.submit_shot(mids[0], s)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_single_short_job_after_autotune(jobs, alone, use_graph):
+    """A lone job whose group is tuned after its first steps (the tuning
+    rebuild follows ticks still in flight) keeps the plain engine's bits."""
+    import rtm_amd
+    print(f"case: lone job 4 with autotuning, graph={use_graph}", flush=True)
+    for _ in range(3):
+        with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+            jid = b.submit(*jobs[4][:5], snapshot_format=jobs[4][5])
+            b.run()
+            I, S = b.result(jid)
+        assert bits_equal(alone[4][0], I)
+        assert bits_equal(alone[4][1], S)
