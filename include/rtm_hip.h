@@ -262,7 +262,10 @@ typedef struct {
     long long ticks;             /* time steps run (each advances every active job) */
     long long rebuilds;          /* launch-table rebuilds (job set changes)         */
     double    cell_steps;        /* 2*nt*nxe*nze summed over finished jobs          */
-    double    run_seconds;       /* wall time inside rtm_hip_batch_run             */
+    double    run_seconds;       /* wall time inside rtm_hip_batch_run, of which: */
+    double    admit_seconds;     /*   admission (allocation, tables, upload)       */
+    double    tune_seconds;      /*   kernel autotuning                            */
+    double    retire_seconds;    /*   image download, stack update, freeing        */
     double    busy_latency_s;    /* sum of submit-to-done times of finished jobs   */
     int       models_resident;   /* registered models (rtm_hip_batch_add_model)     */
     size_t    bytes_in_use, bytes_budget;
@@ -292,6 +295,14 @@ int         rtm_hip_batch_remove_model(rtm_hip_batch* b, int model_id);
  * src/rtm/rtm.cpp:88-101); stack position as in rtm_hip_job. */
 int         rtm_hip_batch_submit_shot(rtm_hip_batch* b, int model_id, const rtm_hip_shot* shot,
                                       int stack_ix0, int stack_iz0, long long* id);
+
+/* Reserves `bytes` of device memory (0: the whole remaining budget) in one
+ * allocation, from which every later job and model buffer is placed; the
+ * budget becomes what is in use plus the reservation.  Admission then costs
+ * no device allocation (a large allocation also clears its memory, which is
+ * the dominant cost of admitting a job otherwise).  Only before the first
+ * submission. */
+int         rtm_hip_batch_reserve(rtm_hip_batch* b, size_t bytes);
 
 /* Device bytes admission needs for the job (its own buffers plus the tables
  * of its private model). */

@@ -24,9 +24,10 @@
 // Launches.  Jobs are grouped by stencil half-width; each group has one
 // launch per tick over the concatenation of its jobs' tiles, with the tile
 // width and cross-lane method picked once per half-width by timing dry runs
-// on the live jobs.  A tick (the group launches plus the clock kernel), and a
-// run of 32 ticks, are captured in hipGraphs and replayed until the job set
-// changes, so one host call advances every job 32 steps.
+// on the live jobs.  A tick (the group launches plus the clock update), and a
+// run of 32 ticks (32 sets of group launches, one clock update), are captured
+// in hipGraphs and replayed until the job set changes, so one host call
+// advances every job 32 steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +42,7 @@
 #include <string>
 #include <vector>
 
+#include "arena.hpp"
 #include "batch_kernels.hpp"
 #include "host_tables.hpp"
 #include "rtm_hip.h"
@@ -120,7 +122,6 @@ struct rtm_hip_batch {
     std::deque<Job*> queue;
     std::vector<Job*> active;
     JobDesc* d_jobs = nullptr;
-    rtm_hip::StepArgs* d_args = nullptr;             // per job, at its clock
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by stencil half-width
     bool dirty = true;
@@ -130,7 +131,22 @@ struct rtm_hip_batch {
     int stack_nx = 0, stack_nz = 0;
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
+    rtm_hip::Arena arena;                            // reserved job memory, if any
     rtm_hip_batch_stats stats{};
+
+    // Job and model buffers: from the reserved arena when there is one.
+    hipError_t dev_alloc(void** p, size_t bytes) {
+        if (arena.active()) {
+            *p = arena.alloc(bytes);
+            return *p ? hipSuccess : hipErrorOutOfMemory;
+        }
+        return hipMalloc(p, std::max<size_t>(bytes, 1));
+    }
+    void dev_free(void* p) {
+        if (!p) return;
+        if (arena.contains(p)) arena.release(p);
+        else (void)hipFree(p);
+    }
 
     int fail(int code, const char* fmt, ...) {
         char buf[1024];
@@ -193,9 +209,8 @@ int model_init(rtm_hip_batch* b, Model& m, const rtm_hip_grid* grid, const float
     return RTM_HIP_OK;
 }
 
-void model_release(Model& m) {
-    for (float* p : {m.vdt2, m.wx, m.wz, m.wavelet})
-        if (p) (void)hipFree(p);
+void model_release(rtm_hip_batch* b, Model& m) {
+    for (float* p : {m.vdt2, m.wx, m.wz, m.wavelet}) b->dev_free(p);
     m.vdt2 = m.wx = m.wz = m.wavelet = nullptr;
     m.resident = false;
 }
@@ -214,11 +229,11 @@ int model_upload(rtm_hip_batch* b, Model& m) {
         const std::vector<float>* src;
     } ups[] = {{&m.vdt2, &t.vdt2}, {&m.wx, &t.wx}, {&m.wz, &t.wz}, {&m.wavelet, &t.wavelet}};
     for (const Up& u : ups) {
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(u.dst), std::max<size_t>(u.src->size(), 1) * sizeof(float));
+        hipError_t e = b->dev_alloc(reinterpret_cast<void**>(u.dst), u.src->size() * sizeof(float));
         if (e == hipSuccess)
             e = hipMemcpy(*u.dst, u.src->data(), u.src->size() * sizeof(float), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            model_release(m);
+            model_release(b, m);
             return b->hip_fail(e, "model tables");
         }
     }
@@ -233,7 +248,7 @@ int model_upload(rtm_hip_batch* b, Model& m) {
 void model_maybe_free(rtm_hip_batch* b, const std::shared_ptr<Model>& m) {
     if (m->refs > 0 || !m->resident) return;
     if (m->registered && !m->removing) return;
-    model_release(*m);
+    model_release(b, *m);
     b->used -= m->bytes;
 }
 
@@ -280,7 +295,7 @@ size_t admission_bytes(const Job& j) { return j.bytes + (j.model->resident ? 0 :
 
 template <typename T>
 int job_alloc(rtm_hip_batch* b, std::vector<void*>& list, T** p, size_t count, const char* what) {
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+    hipError_t e = b->dev_alloc(reinterpret_cast<void**>(p), count * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
         return b->hip_fail(e, what);
@@ -289,15 +304,15 @@ int job_alloc(rtm_hip_batch* b, std::vector<void*>& list, T** p, size_t count, c
     return RTM_HIP_OK;
 }
 
-void free_entries(Job& j) {
-    for (void* p : j.entry_allocs) (void)hipFree(p);
+void free_entries(rtm_hip_batch* b, Job& j) {
+    for (void* p : j.entry_allocs) b->dev_free(p);
     j.entry_allocs.clear();
     j.tile_x_built = 0;
 }
 
-void job_free(Job& j) {
-    free_entries(j);
-    for (void* p : j.allocs) (void)hipFree(p);
+void job_free(rtm_hip_batch* b, Job& j) {
+    free_entries(b, j);
+    for (void* p : j.allocs) b->dev_free(p);
     j.allocs.clear();
 }
 
@@ -313,7 +328,7 @@ void job_cell(const Model& m, float x, float z, int* ixe, int* ize) {
 // Per-tile injection tables of the job for tile width tile_x (forward: the
 // source; backward: every receiver, in receiver order within a cell).
 int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
-    free_entries(j);
+    free_entries(b, j);
     const Model& m = *j.model;
     const rtm_hip::StepGeometry g =
         rtm_hip::step_geometry(rtm_hip::Family::kMarchDpp, m.half, tile_x, m.nxe, m.nze, m.ldz);
@@ -411,12 +426,9 @@ int rebuild(rtm_hip_batch* b) {
     b->drop_graph();
     if (b->active.size() > b->jobs_cap) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
-        if (b->d_args) (void)hipFree(b->d_args);
         b->d_jobs = nullptr;
-        b->d_args = nullptr;
         b->jobs_cap = std::max<size_t>(b->active.size(), 2 * b->jobs_cap);
         hipError_t e = hipMalloc(&b->d_jobs, b->jobs_cap * sizeof(JobDesc));
-        if (e == hipSuccess) e = hipMalloc(&b->d_args, b->jobs_cap * sizeof(rtm_hip::StepArgs));
         if (e != hipSuccess) {
             b->jobs_cap = 0;
             return b->hip_fail(e, "job descriptors");
@@ -453,16 +465,11 @@ int rebuild(rtm_hip_batch* b) {
     // every clock is known on the host: one step per tick since admission
     std::vector<JobDesc> descs;
     for (Job* j : b->active) {
-        const int nt = j->model->nt;
-        const int done = 2 * nt - j->steps_left;
-        j->desc.phase = done / nt;
-        j->desc.step = done % nt;
+        j->desc.done = 2 * j->model->nt - j->steps_left;
         descs.push_back(j->desc);
     }
     if (!descs.empty()) {
         hipError_t e = hipMemcpy(b->d_jobs, descs.data(), descs.size() * sizeof(JobDesc), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)descs.size(), 0, 0, b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "descriptor upload");
     }
     b->dirty = false;
@@ -470,14 +477,18 @@ int rebuild(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
-int enqueue_tick(rtm_hip_batch* b) {
-    for (auto& [half, g] : b->groups) {
-        if (g.ntiles == 0) continue;
-        hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, b->d_args, g.d_tile_job,
-                                                  g.ntiles, b->stream);
-        if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
+// A run of `ticks` ticks: the group launches of every tick, then one clock
+// update for the run.
+int enqueue_ticks(rtm_hip_batch* b, int ticks) {
+    for (int sub = 0; sub < ticks; ++sub) {
+        for (auto& [half, g] : b->groups) {
+            if (g.ntiles == 0) continue;
+            hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job, g.ntiles,
+                                                      sub, 0, b->stream);
+            if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
+        }
     }
-    hipError_t e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 1, 0, b->stream);
+    hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
@@ -514,14 +525,12 @@ int autotune(rtm_hip_batch* b) {
                 g.tile_x = tx;
                 g.family = fam;
                 if ((rc = rebuild(b))) break;
-                e = rtm_hip::launch_batch_clock(b->d_jobs, b->d_args, (int)b->active.size(), 0, 1, b->stream);
-                if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                 // 2 warm steps, then the best of 3 rounds of 4 timed steps
                 float ms = 1e30f;
                 for (int round = 0; round < 3 && !rc; ++round) {
                     for (int rep = 0; rep < (round == 0 ? 6 : 4) && !rc; ++rep) {
                         if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
-                        e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, b->d_args, g.d_tile_job, g.ntiles,
+                        e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, g.d_tile_job, g.ntiles, 0, 1,
                                                        b->stream);
                         if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
                     }
@@ -566,14 +575,18 @@ int admit_from_queue(rtm_hip_batch* b) {
                                "are held by resident models", j->id, need / kGiB, b->used / kGiB, b->budget / kGiB);
             break;
         }
-        b->queue.pop_front();
         if (int rc = admit_job(b, *j)) {
-            job_free(*j);
+            job_free(b, *j);
+            // a reserved arena can be too fragmented for the job while the
+            // budget has room: it waits for running jobs to free memory
+            if (rc == RTM_HIP_ERR_MEMORY && b->arena.active() && !b->active.empty()) break;
+            b->queue.pop_front();
             j->state = RTM_HIP_JOB_FAILED;
             j->model->refs -= 1;
             model_maybe_free(b, j->model);
             return rc;
         }
+        b->queue.pop_front();
         b->used += j->bytes;
         j->state = RTM_HIP_JOB_RUNNING;
         j->t_admit = now_s();
@@ -608,7 +621,7 @@ int retire_finished(rtm_hip_batch* b) {
                                b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) return b->hip_fail(e, "image download");
-        job_free(*j);
+        job_free(b, *j);
         b->used -= j->bytes;
         j->state = RTM_HIP_JOB_DONE;
         j->t_done = now_s();
@@ -627,8 +640,7 @@ int retire_finished(rtm_hip_batch* b) {
 int capture_ticks(rtm_hip_batch* b, int k, int ticks) {
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-    int rc = RTM_HIP_OK;
-    for (int i = 0; i < ticks && !rc; ++i) rc = enqueue_tick(b);
+    const int rc = enqueue_ticks(b, ticks);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -668,8 +680,8 @@ int run_ticks(rtm_hip_batch* b, int n) {
             if (e != hipSuccess) return b->hip_fail(e, "graph launch");
         }
     } else {
-        for (int i = 0; i < n; ++i)
-            if (int rc = enqueue_tick(b)) return rc;
+        for (int left = n; left > 0; left -= kGraphTicks)
+            if (int rc = enqueue_ticks(b, std::min(left, kGraphTicks))) return rc;
     }
     for (Job* j : b->active) j->steps_left -= n;
     b->stats.ticks += n;
@@ -754,17 +766,34 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     b->drop_graph();
-    for (Job* j : b->active) job_free(*j);
-    for (auto& [id, j] : b->jobs) model_release(*j->model);
-    for (auto& [id, m] : b->models) model_release(*m);
+    for (Job* j : b->active) job_free(b, *j);
+    for (auto& [id, j] : b->jobs) model_release(b, *j->model);
+    for (auto& [id, m] : b->models) model_release(b, *m);
     for (auto& [half, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
-    if (b->d_args) (void)hipFree(b->d_args);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
+    if (b->arena.active()) (void)hipFree(b->arena.base());
     (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+int rtm_hip_batch_reserve(rtm_hip_batch* b, size_t bytes) {
+    if (int rc = check(b)) return rc;
+    if (!b->active.empty() || !b->queue.empty())
+        return b->fail(RTM_HIP_ERR_ARG, "reserve memory before submitting jobs");
+    if (b->arena.active()) return b->fail(RTM_HIP_ERR_ARG, "memory is already reserved");
+    if (bytes == 0) bytes = b->budget > b->used ? b->budget - b->used : 0;
+    bytes &= ~size_t(255);
+    if (bytes == 0) return b->fail(RTM_HIP_ERR_MEMORY, "no budget left to reserve");
+    (void)hipSetDevice(b->device);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return b->hip_fail(e, "memory reservation");
+    b->arena.reset(static_cast<char*>(p), bytes);
+    b->budget = b->used + bytes;
+    return RTM_HIP_OK;
 }
 
 const char* rtm_hip_batch_last_error(const rtm_hip_batch* b) {
@@ -836,7 +865,10 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
     long long ran = 0;
     int rc = RTM_HIP_OK;
     for (;;) {
-        if ((rc = admit_from_queue(b))) break;
+        const double ta = now_s();
+        rc = admit_from_queue(b);
+        b->stats.admit_seconds += now_s() - ta;
+        if (rc) break;
         if (b->active.empty()) break;
         if (b->dirty && (rc = rebuild(b))) break;
         // every job moves one step per tick: run until the next one finishes
@@ -853,14 +885,20 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         if (warm > 0) {
             n = std::min(n, warm);
         } else {
-            if ((rc = autotune(b))) break;
+            const double tt = now_s();
+            rc = autotune(b);
+            b->stats.tune_seconds += now_s() - tt;
+            if (rc) break;
             if (b->dirty && (rc = rebuild(b))) break;
         }
         if (max_ticks > 0) n = (int)std::min<long long>(n, max_ticks - ran);
         if (n <= 0) break;
         if ((rc = run_ticks(b, n))) break;
         ran += n;
-        if ((rc = retire_finished(b))) break;
+        const double tr = now_s();
+        rc = retire_finished(b);
+        b->stats.retire_seconds += now_s() - tr;
+        if (rc) break;
         if (!until_done && (max_ticks <= 0 || ran >= max_ticks)) break;
         if (until_done && max_ticks > 0 && ran >= max_ticks) break;
     }

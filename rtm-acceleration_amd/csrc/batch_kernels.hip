@@ -24,11 +24,14 @@ __device__ __forceinline__ float* rotate(const JobDesc& J, int r) {
     return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
 }
 
-// Step arguments of job J at its current clock.
-__device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
+// Step arguments of job J after `total` of its 2 nt steps (forward steps
+// 0..nt-1, then backward steps 0..nt-1).  The stencil coefficients are read
+// from the descriptor where they are used (an array copied into a local
+// struct would be kept in scratch memory).
+__device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dry) {
     StepArgs a{};
-    const int fwd = J.phase == 0;
-    const int s = J.step;
+    const int fwd = total < J.nt;
+    const int s = fwd ? total : total - J.nt;
     const int base = fwd ? 0 : J.bwd_base;
     a.p_prev = rotate(J, (base + s) % 3);
     a.p_cur = rotate(J, (base + s + 1) % 3);
@@ -38,13 +41,12 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
     a.vdt2 = J.vdt2;
     a.wx = J.wx;
     a.wz = J.wz;
-    a.coef = J.coef;
     a.nxe = J.nxe; a.nze = J.nze; a.ldz = J.ldz; a.nb = J.nb; a.nx = J.nx; a.nz = J.nz;
     a.ntz = J.ntz; a.ntx = J.ntx; a.ntiles = J.ntiles;
     const int it = fwd ? s : J.nt - 1 - s;
     a.it = it;
-    a.tile_ptr = J.tile_ptr[fwd ? 0 : 1];
-    a.entries = J.entries[fwd ? 0 : 1];
+    a.tile_ptr = fwd ? J.tile_ptr[0] : J.tile_ptr[1];
+    a.entries = fwd ? J.entries[0] : J.entries[1];
     a.wavelet = J.wavelet;
     a.traces = J.traces;
     a.trace_stride = J.trace_stride;
@@ -52,72 +54,60 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int dry) {
     a.illum = J.illum;
     a.dry = dry;
     a.snap_fmt = J.snap_fmt;
+    // Every field is assigned once, unconditionally: stores of different
+    // fields in different branches can be merged into one store at a
+    // computed offset, which would keep the whole struct in scratch memory.
     const bool stored = (it % J.store) == 0;
     const long long slot = it / J.store;
-    if (J.snap_fmt == (int)SnapFmt::F32) {
-        if (stored) {
-            if (fwd) a.snap_out = J.snap32 + slot * J.n_int;
-            else a.snap_in = J.snap32 + slot * J.n_int;
-        }
-    } else if (fwd) {
-        if (stored) a.snap_max = reinterpret_cast<unsigned*>(J.snap_max + slot);
-        if (it >= 1 && ((it - 1) % J.store) == 0) {
-            const long long prev = (it - 1) / J.store;
-            a.quant_out = J.snap16 + prev * J.n_int;
-            a.quant_max = J.snap_max + prev;
-        }
-    } else if (stored) {
-        a.snap16_max = J.snap_max + slot;
-        if (s == 0) a.pending_field = rotate(J, J.bwd_base);   // last forward field
-        else a.snap16_in = J.snap16 + slot * J.n_int;
-    }
+    const bool wide = J.snap_fmt == (int)SnapFmt::F32;
+    const bool quant = !wide && fwd && it >= 1 && ((it - 1) % J.store) == 0;
+    const long long prev = quant ? (it - 1) / J.store : 0;
+    a.snap_out = (wide && fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
+    a.snap_in = (wide && !fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
+    a.snap_max = (!wide && fwd && stored) ? reinterpret_cast<unsigned*>(J.snap_max + slot) : nullptr;
+    a.quant_out = quant ? J.snap16 + prev * J.n_int : nullptr;
+    a.quant_max = quant ? J.snap_max + prev : nullptr;
+    a.snap16_max = (!wide && !fwd && stored) ? J.snap_max + slot : nullptr;
+    // the last forward field, at backward step 0
+    a.pending_field = (!wide && !fwd && stored && s == 0) ? rotate(J, J.bwd_base) : nullptr;
+    a.snap16_in = (!wide && !fwd && stored && s != 0) ? J.snap16 + slot * J.n_int : nullptr;
     return a;
 }
 
-// One tick of every job of half-width H.  The job and tile of a wave are
-// wave-uniform (read-first-lane), so the job's step arguments, prepared by
-// k_clock, are read with scalar loads as a kernel argument would be.
+// Tick `sub` of a run of ticks for every job of half-width H: job J is at
+// step J.done + sub.  The job and tile of a wave are wave-uniform
+// (read-first-lane), so the descriptor is read with scalar loads and the
+// step arguments are built in scalar registers, as a kernel argument would
+// be.
 template <int H, int TX, Family F>
 __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
-                                                         const StepArgs* __restrict__ args,
-                                                         const int* __restrict__ tile_job, int ntiles) {
+                                                         const int* __restrict__ tile_job, int ntiles, int sub,
+                                                         int dry) {
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (ntiles + kWaves - 1) / kWaves;
     const int g = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (g >= ntiles) return;
     const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
-    const int phase = jobs[jid].phase;
-    if (phase > 1) return;                   // finished job: its tiles idle
-    const int t = g - jobs[jid].tile_first;
+    const JobDesc& J = jobs[jid];
+    const int total = J.done + sub;
+    if (total >= 2 * J.nt) return;           // finished job: its tiles idle
+    const StepArgs a = job_args(J, total, dry);
+    const int t = g - J.tile_first;
     if constexpr (F == Family::kStream) {
-        if (phase == 0) stream_tile<H, TX, Dir::Forward>(args[jid], t, lane);
-        else stream_tile<H, TX, Dir::Backward>(args[jid], t, lane);
+        if (total < J.nt) stream_tile<H, TX, Dir::Forward>(a, J.coef, t, lane);
+        else stream_tile<H, TX, Dir::Backward>(a, J.coef, t, lane);
     } else {
         constexpr bool DPP = F == Family::kMarchDpp;
-        if (phase == 0) march_tile<H, TX, Dir::Forward, DPP>(args[jid], t, lane);
-        else march_tile<H, TX, Dir::Backward, DPP>(args[jid], t, lane);
+        if (total < J.nt) march_tile<H, TX, Dir::Forward, DPP>(a, J.coef, t, lane);
+        else march_tile<H, TX, Dir::Backward, DPP>(a, J.coef, t, lane);
     }
 }
 
-// Job clocks (advance != 0: one step; at nt, the next phase) and the step
-// arguments of every job at its clock.
-__global__ __launch_bounds__(64) void k_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry) {
+// Job clocks: every job has done `ticks` more steps.
+__global__ __launch_bounds__(64) void k_advance(JobDesc* jobs, int njobs, int ticks) {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= njobs) return;
-    JobDesc& J = jobs[j];
-    if (J.phase > 1) return;
-    if (advance) {
-        const int s = J.step + 1;
-        if (s == J.nt) {
-            J.step = 0;
-            J.phase = J.phase + 1;
-        } else {
-            J.step = s;
-        }
-        if (J.phase > 1) return;
-    }
-    args[j] = job_args(J, dry);
+    if (j < njobs) jobs[j].done += ticks;
 }
 
 __global__ __launch_bounds__(256) void k_stack_add(const float* image, const float* illum, int nx, int nz,
@@ -131,11 +121,12 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 }
 
 template <int H>
-hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const StepArgs* args, const int* tile_job,
-                    int ntiles, hipStream_t s) {
+hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* tile_job, int ntiles, int sub,
+                    int dry, hipStream_t s) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
-#define RTM_BATCH(TX, F) hipLaunchKernelGGL((k_batch<H, TX, F>), grid, block, 0, s, jobs, args, tile_job, ntiles)
+#define RTM_BATCH(TX, F) \
+    hipLaunchKernelGGL((k_batch<H, TX, F>), grid, block, 0, s, jobs, tile_job, ntiles, sub, dry)
     if (family == Family::kStream) {
         switch (tile_x) {
             case 8: RTM_BATCH(8, Family::kStream); break;
@@ -168,16 +159,16 @@ hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const StepAr
 
 }  // namespace
 
-hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const StepArgs* args,
-                             const int* tile_job, int ntiles, hipStream_t stream) {
+hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
+                             int ntiles, int sub, int dry, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1>(family, tile_x, jobs, args, tile_job, ntiles, stream);
-        case 2: return launch_h<2>(family, tile_x, jobs, args, tile_job, ntiles, stream);
-        case 3: return launch_h<3>(family, tile_x, jobs, args, tile_job, ntiles, stream);
-        case 4: return launch_h<4>(family, tile_x, jobs, args, tile_job, ntiles, stream);
-        case 5: return launch_h<5>(family, tile_x, jobs, args, tile_job, ntiles, stream);
-        case 6: return launch_h<6>(family, tile_x, jobs, args, tile_job, ntiles, stream);
+        case 1: return launch_h<1>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 2: return launch_h<2>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 3: return launch_h<3>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 4: return launch_h<4>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 5: return launch_h<5>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 6: return launch_h<6>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -190,10 +181,9 @@ hipError_t launch_stack_add(const float* image, const float* illum, int nx, int 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry, hipStream_t stream) {
+hipError_t launch_batch_advance(JobDesc* jobs, int njobs, int ticks, hipStream_t stream) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_clock, dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, stream, jobs, args, njobs,
-                       advance, dry);
+    hipLaunchKernelGGL(k_advance, dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, stream, jobs, njobs, ticks);
     return hipGetLastError();
 }
 

@@ -1,13 +1,13 @@
 // batch_kernels.hpp -- many shots, one launch per time step.
 //
 // Each job is one shot on its own grid (its own nx, nz, order, nt, store
-// interval and tables).  The device holds one descriptor per job; a tick
-// advances every active job by one step of its current phase (forward, then
-// backward with imaging) with one launch per stencil half-width present, and
-// a one-thread-per-job kernel then moves the job clocks and prepares every
-// job's step arguments for the next tick.  The launch
-// arguments never change between ticks, so a tick can be captured once in a
-// hipGraph and replayed.
+// interval and tables).  The device holds one descriptor per job with the
+// number of steps it has done; a tick advances every active job by one step
+// of its current phase (forward, then backward with imaging) with one launch
+// per stencil half-width present.  A run of k ticks passes each launch its
+// offset 0..k-1 within the run and ends with one launch that adds k to every
+// job's count, so the launches of a run never change and a run can be
+// captured once in a hipGraph and replayed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,9 +23,8 @@ struct JobDesc {
     int ntz, ntx, ntiles;        // this job's tile grid (march mapping)
     int tile_first;              // its first tile in the group launch
     int bwd_base;                // buffer rotation base of the backward phase
-    // clock, advanced on the device after every tick
-    int phase;                   // 0 forward, 1 backward, 2 done
-    int step;                    // steps completed in the phase
+    // clock, advanced on the device after every run of ticks
+    int done;                    // steps done of 2 nt (forward nt, then backward nt)
     Coeffs coef;
     long long trace_stride;
     long long n_int;
@@ -44,23 +43,20 @@ struct JobDesc {
     const Entry* entries[2];
 };
 
-// One tick for the jobs of one half-width H: tile_job[g] names the job of
-// global tile g, args[j] holds job j's step arguments at its clock (written
-// by launch_batch_clock); family kMarch, kMarchDpp or kStream with the tile
-// widths of the per-shot kernels.
-hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const StepArgs* args,
-                             const int* tile_job, int ntiles, hipStream_t stream);
+// Tick `sub` of a run for the jobs of one half-width H: tile_job[g] names
+// the job of global tile g, and job j computes its step jobs[j].done + sub;
+// family kMarch, kMarchDpp or kStream with the tile widths of the per-shot
+// kernels.  dry != 0 writes only p^{n+1}: no image, illumination or snapshot
+// update (used to time candidate kernels on live state).
+hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
+                             int ntiles, int sub, int dry, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.
 hipError_t launch_stack_add(const float* image, const float* illum, int nx, int nz, double* stack_i,
                             double* stack_s, int stack_nz, int ix0, int iz0, hipStream_t stream);
 
-// Job clocks and step arguments: advance != 0 moves every clock one step
-// (at nt, to the next phase) first; then args[j] = the step arguments of job
-// j at its clock, with the dry flag given (dry steps write only p^{n+1}: no
-// image, illumination or snapshot update; used to time candidate kernels on
-// live state).
-hipError_t launch_batch_clock(JobDesc* jobs, StepArgs* args, int njobs, int advance, int dry, hipStream_t stream);
+// Job clocks: every job has done `ticks` more steps.
+hipError_t launch_batch_advance(JobDesc* jobs, int njobs, int ticks, hipStream_t stream);
 
 }  // namespace rtm_hip

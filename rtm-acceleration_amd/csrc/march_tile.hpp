@@ -115,15 +115,15 @@ __device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx
 // Pieces shared by the march and stream mappings.
 // ---------------------------------------------------------------------------
 
-// Laplacian at the lane's cell of column c of the p^n window q (q[c-H..c+H]
+// Laplacian (coefficients cf) at the lane's cell of column c of the p^n window q (q[c-H..c+H]
 // hold columns x-H..x+H): c0 p + sum_k (cx_k (x+k + x-k) + cz_k (z+k + z-k)),
 // summed in the reference's order (src/rtm/forward.cpp:38-44).  Depth
 // neighbours come from other lanes: chained one-lane DPP shifts, or
 // ds_bpermute (__shfl_up/__shfl_down) by k lanes.
 template <int H, bool DPP>
-__device__ __forceinline__ float laplacian(const StepArgs& a, const float* q, int c) {
+__device__ __forceinline__ float laplacian(const Coeffs& cf, const float* q, int c) {
     const float pc = q[c];
-    float lap = a.coef.c0 * pc;
+    float lap = cf.c0 * pc;
     float zp = pc, zm = pc;
 #pragma unroll
     for (int k = 1; k <= H; ++k) {
@@ -134,8 +134,8 @@ __device__ __forceinline__ float laplacian(const StepArgs& a, const float* q, in
             zp = __shfl_down(pc, (unsigned)k, kTileZ);
             zm = __shfl_up(pc, (unsigned)k, kTileZ);
         }
-        const float tx_k = a.coef.cx[k] * (q[c + k] + q[c - k]);
-        const float tz_k = a.coef.cz[k] * (zp + zm);
+        const float tx_k = cf.cx[k] * (q[c + k] + q[c - k]);
+        const float tz_k = cf.cz[k] * (zp + zm);
         lap = lap + (tx_k + tz_k);
     }
     return lap;
@@ -221,7 +221,7 @@ __device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
 template <int H, int TX, Dir D, bool DPP>
-__device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
+__device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
 
@@ -246,7 +246,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
 
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
-        const float lap = laplacian<H, DPP>(a, q, r + H);
+        const float lap = laplacian<H, DPP>(cf, q, r + H);
         const int x = x0 + r;
         if (!z_out || x >= a.nxe - H) continue;
         finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, e0, e1, vmax);
@@ -264,7 +264,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, int t, int lane) {
 // own entries.
 // ---------------------------------------------------------------------------
 template <int H, int XS, Dir D>
-__device__ __forceinline__ void stream_tile(const StepArgs& a, int t, int lane) {
+__device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int C = 4;                     // columns per chunk
     constexpr int NCH = XS / C;
     constexpr int W = C + 2 * H;             // p^n window of one chunk
@@ -310,7 +310,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, int t, int lane) 
             int ee = ec;
             const long long col_end = (long long)(x + 1) * a.ldz;
             while (ee < e1 && gbl(a.entries)[ee].cell < col_end) ++ee;
-            const float lap = laplacian<H, true>(a, q, r + H);
+            const float lap = laplacian<H, true>(cf, q, r + H);
             if (z_out && x < a.nxe - H)
                 finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, ec, ee, vmax);
             ec = ee;

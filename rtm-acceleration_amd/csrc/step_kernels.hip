@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (t >= a.ntiles) return;               // whole wave leaves together
-    march_tile<H, TX, D, DPP>(a, t, lane);
+    march_tile<H, TX, D, DPP>(a, a.coef, t, lane);
 }
 
 // Stream mapping: one wave per strip of XS columns (march_tile.hpp).
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_stream(StepArgs a) {
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (t >= a.ntiles) return;               // whole wave leaves together
-    stream_tile<H, XS, D>(a, t, lane);
+    stream_tile<H, XS, D>(a, a.coef, t, lane);
 }
 
 // Quantise the interior of an extended field into a 16-bit snapshot.

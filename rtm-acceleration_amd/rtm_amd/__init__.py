@@ -238,6 +238,10 @@ class BatchEngine:
         j.stack_ix0, j.stack_iz0 = (-1, -1) if stack_at is None else (int(stack_at[0]), int(stack_at[1]))
         return j
 
+    def reserve(self, nbytes: int = 0):
+        """Reserve device memory for all later jobs (0: the whole budget)."""
+        self._check(self.lib.rtm_hip_batch_reserve(self.h, int(nbytes)))
+
     def add_model(self, model: VelocityModel, params: RTMParams, nt: int, dt: float,
                   snapshot_format: int = SNAP_F32) -> int:
         """Resident model (the arguments of HipEngine.setup); returns its id."""

@@ -72,6 +72,7 @@ class BatchStats(C.Structure):
     _fields_ = [("jobs_submitted", C.c_longlong), ("jobs_done", C.c_longlong),
                 ("jobs_active", C.c_int), ("jobs_queued", C.c_int), ("ticks", C.c_longlong),
                 ("rebuilds", C.c_longlong), ("cell_steps", C.c_double), ("run_seconds", C.c_double),
+                ("admit_seconds", C.c_double), ("tune_seconds", C.c_double), ("retire_seconds", C.c_double),
                 ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
                 ("group_tile_x", C.c_int * 6), ("group_family", C.c_int * 6)]
@@ -108,6 +109,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_int)]),
     "rtm_hip_batch_remove_model": (C.c_int, [_H, C.c_int]),
     "rtm_hip_batch_submit_shot": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
+    "rtm_hip_batch_reserve": (C.c_int, [_H, C.c_size_t]),
     "rtm_hip_batch_job_bytes": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_size_t)]),
     "rtm_hip_batch_submit": (C.c_int, [_H, C.POINTER(Job), C.POINTER(C.c_longlong)]),
     "rtm_hip_batch_run": (C.c_int, [_H, C.c_int, C.c_int]),

@@ -106,11 +106,15 @@ def served_stream(specs: dict[str, ModelSpec], rate: float, njobs: int, seed: in
 
 
 def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0, chunk_s: float = 0.004,
-          use_graph: bool = True, memory_budget: int = 0, keep_images: bool = False) -> dict:
+          use_graph: bool = True, memory_budget: int = 0, keep_images: bool = False, reserve: int = 0) -> dict:
     """Runs the stream through one BatchEngine in real time; fills each
-    request's latency (and image when keep_images).  Returns a summary."""
+    request's latency (and image when keep_images).  reserve > 0 places all
+    job memory in one block of that many bytes, reserved before the stream
+    starts (-1: the whole budget).  Returns a summary."""
     with BatchEngine(device, memory_budget=memory_budget, use_graph=use_graph) as b:
         mids = {k: b.add_model(sp.model, sp.params, sp.nt, sp.dt, sp.snapshot_format) for k, sp in specs.items()}
+        if reserve:
+            b.reserve(max(reserve, 0))     # the server's job memory, placed before requests arrive
         pending = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
         out: dict[int, tuple[int, float]] = {}     # job id -> (request index, submit lag)
         nxt = 0

@@ -237,3 +237,26 @@ def test_single_short_job_after_autotune(jobs, alone, use_graph):
             I, S = b.result(jid)
         assert bits_equal(alone[4][0], I)
         assert bits_equal(alone[4][1], S)
+
+
+def test_reserved_arena(jobs, alone):
+    """Jobs placed in a reserved block: a block that holds only a few jobs at
+    a time makes later jobs wait for memory (and for a free run of it), and
+    every image still equals the plain engine's."""
+    import rtm_amd
+    print("case: batch in a reserved arena", flush=True)
+    with rtm_amd.BatchEngine(0) as probe:
+        sizes = sorted(probe.job_bytes(m, p, nt, dt, s, f) for (m, p, nt, dt, s, f) in jobs)
+    for reserve in (int(sum(sizes[-3:]) * 1.05), 1 << 30):
+        with rtm_amd.BatchEngine(0) as b:
+            b.reserve(reserve)
+            assert b.stats()["bytes_budget"] <= reserve
+            ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+            with pytest.raises(RuntimeError, match="before submitting"):
+                b.reserve(1 << 20)
+            b.run()
+            assert b.stats()["bytes_in_use"] == 0
+            for k, jid in enumerate(ids):
+                I, S = b.result(jid)
+                assert bits_equal(alone[k][0], I), f"job {k}, reserve {reserve}"
+                assert bits_equal(alone[k][1], S), f"job {k}, reserve {reserve}"
