@@ -67,3 +67,35 @@ def test_full_size_properties(hip_engine_cls, workload):
     assert np.abs(I1).max() > 0
     assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(2.0 * I1)
     assert np.abs(d).max() <= 1e-4 * 2.0 * np.abs(I1).max()
+
+
+def test_crops_batched_equal_alone(hip_engine_cls):
+    """Aperture crops of the 2301 x 751 model at their real sizes (widths
+    U[400,1200], nz 751, orders 8 and 12, seeded; nt shortened to 600-1200 to
+    keep the test short), all in one batch with FP32 and FP16 snapshots: every
+    image equals the plain engine's migration of that crop alone, bit for bit."""
+    import rtm_amd
+    from rtm_amd import serve
+    specs, reqs = serve.crop_batch(6, seed=7)
+    rng = np.random.default_rng(7)
+    for k, key in enumerate(sorted(specs)):
+        sp = specs[key]
+        sp.nt = int(rng.integers(600, 1201))
+        sp.snapshot_format = rtm_amd.SNAP_F16 if k % 2 else rtm_amd.SNAP_F32
+        s = sp.shots[0]
+        sp.shots[0] = rtm_amd.ShotRecord(s.sx, s.sz, s.rx, s.rz, sp.nt, s.dt)
+    print(f"case: {len(specs)} crops batched vs alone", flush=True)
+    alone = {}
+    for key, sp in specs.items():
+        with hip_engine_cls() as eng:
+            eng.setup(sp.model, sp.params, sp.nt, sp.dt, snapshot_format=sp.snapshot_format)
+            sp.shots[0].traces = eng.model_shot(sp.shots[0])
+            alone[key] = eng.migrate([sp.shots[0]])
+    with rtm_amd.BatchEngine(0) as b:
+        ids = {key: b.submit(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
+               for key, sp in specs.items()}
+        b.run()
+        for key, jid in ids.items():
+            I, S = b.result(jid)
+            assert bits_equal(alone[key][0], I), key
+            assert bits_equal(alone[key][1], S), key
