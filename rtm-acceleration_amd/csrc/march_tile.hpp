@@ -59,20 +59,64 @@ __device__ __forceinline__ uint16_t encode_snapshot(float v, int k, int fmt) {
     return bits;
 }
 
-// Source / receiver injection and trace recording of one output cell, in
-// entry order (the tile's entries are sorted by cell, then code).
+// The first 64 injection entries of a tile, one per lane, with their
+// amplitude at this step (source: wavelet[it]; receiver, backward: its trace
+// sample), loaded at the start of the tile together with its other operands.
+// The epilogue then reads them with v_readlane (no memory round trip after
+// the stencil); entries past the first 64 are read from memory there.
+struct TileEntries {
+    int base;          // the tile's first entry
+    int cell, code;    // entry base + lane (cell -1 past the tile's entries)
+    float amp;
+};
+
 template <Dir D>
-__device__ __forceinline__ float apply_entries(const StepArgs& a, int cell, float out, float v2, int e0,
-                                               int e1) {
-    for (int e = e0; e < e1; ++e) {
-        const int en_cell = gbl(a.entries)[e].cell;
-        const int en_code = gbl(a.entries)[e].code;
+__device__ __forceinline__ TileEntries load_entries(const StepArgs& a, int e0, int e1, int lane) {
+    TileEntries te;
+    te.base = e0;
+    te.cell = -1;
+    te.code = 0;
+    te.amp = 0.0f;
+    if (e0 + lane < e1) {
+        te.cell = gbl(a.entries)[e0 + lane].cell;
+        te.code = gbl(a.entries)[e0 + lane].code;
+        if constexpr (D == Dir::Backward) te.amp = gbl(a.traces)[(ptrdiff_t)te.code * a.trace_stride + a.it];
+        else if (te.code < 0) te.amp = gbl(a.wavelet)[a.it];
+    }
+    return te;
+}
+
+__device__ __forceinline__ int entry_cell(const StepArgs& a, const TileEntries& te, int e) {
+    const int k = e - te.base;
+    return k < kTileZ ? __builtin_amdgcn_readlane(te.cell, k) : gbl(a.entries)[e].cell;
+}
+
+// Source / receiver injection and trace recording of one output cell, in
+// entry order (the tile's entries are sorted by cell, then code): entries
+// [lo, hi) of the tile are the ones that may hit this cell.
+template <Dir D>
+__device__ __forceinline__ float apply_entries(const StepArgs& a, const TileEntries& te, int cell, float out,
+                                               float v2, int lo, int hi) {
+    for (int e = lo; e < hi; ++e) {
+        const int k = e - te.base;
+        int en_cell, en_code;
+        float amp;
+        if (k < kTileZ) {
+            en_cell = __builtin_amdgcn_readlane(te.cell, k);
+            en_code = __builtin_amdgcn_readlane(te.code, k);
+            amp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(te.amp), k));
+        } else {
+            en_cell = gbl(a.entries)[e].cell;
+            en_code = gbl(a.entries)[e].code;
+            if constexpr (D == Dir::Backward) amp = gbl(a.traces)[(ptrdiff_t)en_code * a.trace_stride + a.it];
+            else amp = en_code < 0 ? gbl(a.wavelet)[a.it] : 0.0f;
+        }
         if (en_cell != cell) continue;
         if constexpr (D == Dir::Forward) {
-            if (en_code < 0) out = out + v2 * gbl(a.wavelet)[a.it];
+            if (en_code < 0) out = out + v2 * amp;
             else gbl(a.rec_out)[(ptrdiff_t)en_code * a.rec_stride + a.it] = out;
         } else {
-            out = out + v2 * gbl(a.traces)[(ptrdiff_t)en_code * a.trace_stride + a.it];
+            out = out + v2 * amp;
         }
     }
     return out;
@@ -183,14 +227,14 @@ __device__ __forceinline__ float load_cur(const StepArgs& a, int x, int z, bool 
 // inject-then-damp).  Entries [e0, e1) may hit this cell.
 template <Dir D>
 __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
-                                            float pc, float lap, const ColumnOps& o, float wzv, int e0, int e1,
-                                            float& vmax) {
+                                            float pc, float lap, const ColumnOps& o, float wzv,
+                                            const TileEntries& te, int e0, int e1, float& vmax) {
     const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
     const float w = o.wx * wzv;
     const float prev = o.pv * w;
     float out = (2.0f * pc - prev) + o.vv * lap;
     out = out * w;
-    out = apply_entries<D>(a, (int)i, out, o.vv, e0, e1);
+    out = apply_entries<D>(a, te, (int)i, out, o.vv, e0, e1);
     gbl(a.p_next)[i] = out;
 
     const int jx = x - a.nb;
@@ -242,6 +286,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
     const int e0 = gbl(a.tile_ptr)[t], e1 = gbl(a.tile_ptr)[t + 1];
+    const TileEntries te = load_entries<D>(a, e0, e1, lane);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -249,7 +294,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         const float lap = laplacian<H, DPP>(cf, q, r + H);
         const int x = x0 + r;
         if (!z_out || x >= a.nxe - H) continue;
-        finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, e0, e1, vmax);
+        finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, e0, e1, vmax);
     }
     finish_tile(a, sc, vmax, lane);
 }
@@ -289,6 +334,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
     int ec = gbl(a.tile_ptr)[t];
     const int e1 = gbl(a.tile_ptr)[t + 1];
+    const TileEntries te = load_entries<D>(a, ec, e1, lane);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -309,10 +355,10 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
             // entries of column x: cells [x ldz, (x+1) ldz)
             int ee = ec;
             const long long col_end = (long long)(x + 1) * a.ldz;
-            while (ee < e1 && gbl(a.entries)[ee].cell < col_end) ++ee;
+            while (ee < e1 && entry_cell(a, te, ee) < col_end) ++ee;
             const float lap = laplacian<H, true>(cf, q, r + H);
             if (z_out && x < a.nxe - H)
-                finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, ec, ee, vmax);
+                finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, ec, ee, vmax);
             ec = ee;
         }
         if (k + 1 < NCH) {

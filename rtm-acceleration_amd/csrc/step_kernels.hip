@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 
     const float wzv = a.wz[iz];
     const int e0 = a.tile_ptr[t], e1 = a.tile_ptr[t + 1];
+    const TileEntries te = load_entries<D>(a, e0, e1, lane);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         const float prev = pv[r] * w;
         float out = (2.0f * pc - prev) + vv[r] * lap;
         out = out * w;
-        out = apply_entries<D>(a, (int)i, out, vv[r], e0, e1);
+        out = apply_entries<D>(a, te, (int)i, out, vv[r], e0, e1);
         a.p_next[i] = out;
 
         const int jx = ix - a.nb;
