@@ -244,6 +244,60 @@ def served_workload(args):
     }
 
 
+def salt_workload(args):
+    """Reduced-precision snapshots on the 4001x1201 salt model: one shot with
+    FP32 snapshots and FP32 stack, one with FP16 snapshots (per-snapshot
+    scale) and the FP64 stack, on the plain engine."""
+    import rtm_amd
+    from rtm_amd import models
+    model = models.salt_body(4001, 1201, 5.0, 5.0)
+    params = rtm_amd.RTMParams(nb=80, order=12, store_interval=4, f0=12.0)
+    nt, dt = args.salt_nt, 0.0004
+    shot = models.surface_shots(model, nt, dt, [0.5 * (model.nx - 1) * model.dx], rec_dx=10.0)[0]
+    nxe, nze = model.nx + 2 * params.nb, model.nz + 2 * params.nb
+    cells = 2.0 * nt * nxe * nze
+    out = {}
+    for name, fmt, stack in (("f32", rtm_amd.SNAP_F32, rtm_amd.STACK_F32),
+                             ("f16", rtm_amd.SNAP_F16, rtm_amd.STACK_F64)):
+        with rtm_amd.HipEngine(0) as eng:
+            eng.setup(model, params, nt, dt, snapshot_format=fmt, image_stack=stack)
+            eng.set_step_kernel(args.family, args.tile)
+            print(f"bench: salt {name}: modelling", file=sys.stderr, flush=True)
+            if shot.traces is None:
+                shot.traces = eng.model_shot(shot)
+            eng.migrate([shot])                       # warm-up
+            eng.set_kernel_timing(-1)
+            eng.reset_times()
+            t0 = time.perf_counter()
+            eng.migrate([shot])
+            el = time.perf_counter() - t0
+            ks = eng.kernel_stats()
+            times = eng.times()
+            info = eng.info()
+        sb = 4 if fmt == rtm_amd.SNAP_F32 else 2
+        _, per_shot, fwd_b, bwd_b = algorithmic_bytes(model, params, nt, sb)
+        bwd_us = ks["sampled_ms"][1] / max(ks["sampled"][1], 1) * 1e3
+        fwd_us = ks["sampled_ms"][0] / max(ks["sampled"][0], 1) * 1e3
+        out[name] = {"value": round(cells / el * 1e-9, 3), "shot_s": round(el, 4),
+                     "forward_kernel_us": round(fwd_us, 3),
+                     "engine_times_s": {k: round(v, 4) for k, v in times.items() if v},
+                     "snapshot_gb": round(info["snapshot_bytes"] / 1e9, 2),
+                     "backward_kernel_us": round(bwd_us, 3),
+                     "backward_bytes_per_launch": round(bwd_b),
+                     "backward_frac_of_peak": round(bwd_b / (bwd_us * 1e-6) * 1e-9 / HBM_PEAK_GBS, 4),
+                     "whole_shot_hbm_frac": round(per_shot / el * 1e-9 / HBM_PEAK_GBS, 4)}
+    return {
+        "metric": METRIC, "value": out["f16"]["value"], "unit": "Gcell-steps/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(out["f16"]["shot_s"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded salt-body model; traces modelled on the GPU)",
+        "config": {"workload": f"4001x1201 @5 m salt model, nt {nt}, dt 0.4 ms, 12 Hz, order 12, nb 80, "
+                               "store 4, one surface shot; FP16 snapshots + FP64 stack (value) vs FP32",
+                   "engine": "hip"},
+        "snapshots": out, "speedup_f16_vs_f32": round(out["f32"]["shot_s"] / out["f16"]["shot_s"], 3),
+    }
+
+
 def _round(d):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
 
@@ -265,10 +319,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nt", type=int, default=8000, help="time steps per shot (8000 = the workload)")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30, help="copy-probe buffer size")
-    ap.add_argument("--workload", choices=["plain", "crops", "served"], default="plain",
+    ap.add_argument("--salt-nt", type=int, default=10000, help="time steps of the salt workload")
+    ap.add_argument("--workload", choices=["plain", "crops", "served", "salt"], default="plain",
                     help="plain: the 2301x751 per-shot line (default), with the batch engine on the same "
-                         "shots; crops: 64 aperture crops batched vs one by one; served: Poisson streams")
+                         "shots; crops: 64 aperture crops batched vs one by one; served: Poisson streams; "
+                         "salt: 4001x1201 order 12 with FP16 vs FP32 snapshots")
     ap.add_argument("--no-batched", action="store_true", help="plain workload without the batched comparison")
+    ap.add_argument("--batched-shots", type=int, default=8,
+                    help="shots the batch engine migrates at once in the plain workload's comparison")
     ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
     ap.add_argument("--rates", default="2,8,32,64", help="arrival rates (1/s) of the served workload")
     ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
@@ -284,18 +342,19 @@ def main():
     if rtm_amd.device_count() < 1:
         sys.exit("bench: no HIP device")
     if args.workload != "plain":
-        line = crops_workload(args) if args.workload == "crops" else served_workload(args)
+        line = {"crops": crops_workload, "served": served_workload, "salt": salt_workload}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     K, W = args.steps, args.warmup
-    model, params, nt, dt, shots = workload(K + W, args.nt)
+    NB = 0 if args.no_batched else max(K, args.batched_shots)
+    model, params, nt, dt, shots = workload(max(K, NB) + W, args.nt)
     copy_peak = rtm_amd.bandwidth_probe(0, args.probe_bytes, 5)
     n_ext, bytes_shot, fwd_b, bwd_b = algorithmic_bytes(model, params, nt)
 
     eng = rtm_amd.HipEngine(0)
     eng.setup(model, params, nt, dt)
     eng.set_step_kernel(args.family, args.tile)
-    print(f"bench: device {eng.info()['device_name']}, modelling {K + W} shots", file=sys.stderr, flush=True)
+    print(f"bench: device {eng.info()['device_name']}, modelling {len(shots)} shots", file=sys.stderr, flush=True)
     for s in shots:
         s.traces = eng.model_shot(s)
 
@@ -353,7 +412,7 @@ def main():
     }
     if not args.no_batched:
         print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
-        bt = batched_same_shots(model, params, nt, dt, shots[W:W + K], shots[0], use_graph=not args.batch_no_graph)
+        bt = batched_same_shots(model, params, nt, dt, shots[W:W + NB], shots[0], use_graph=not args.batch_no_graph)
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
     if not args.no_cpu:

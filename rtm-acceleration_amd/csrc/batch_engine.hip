@@ -124,6 +124,7 @@ struct rtm_hip_batch {
     JobDesc* d_jobs = nullptr;
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by stencil half-width
+    bool any_16bit = false;                          // an active job stores 16-bit snapshots
     bool dirty = true;
     // captured ticks: graph[0] = one tick, graph[1] = kGraphTicks ticks
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -254,13 +255,22 @@ void model_maybe_free(rtm_hip_batch* b, const std::shared_ptr<Model>& m) {
 
 // ---- jobs -----------------------------------------------------------------
 
+// Tiles of the job's largest batched launch (the narrowest march tile).
+size_t max_job_tiles(const Model& m) {
+    size_t n = 0;
+    for (auto f : {rtm_hip::Family::kMarch, rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream})
+        for (int tx : rtm_hip::tile_widths(f))
+            n = std::max(n, (size_t)rtm_hip::step_geometry(f, m.half, tx, m.nxe, m.nze, m.ldz).ntiles);
+    return n;
+}
+
 size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
     const size_t snap_elem = m.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
     const size_t tiles_bound = (size_t)(m.nxe / 4 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 4, strip >= 52
     size_t bytes = 0;
     bytes += 3 * m.n_fld * sizeof(float);                   // three wavefields
     bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store
-    bytes += (size_t)m.nsnap * sizeof(float);               // snapshot scales
+    bytes += ((size_t)m.nsnap + tiles_bound) * sizeof(float);   // snapshot scales, per-tile maxima
     bytes += 2 * m.n_int * sizeof(float);                   // image, illumination
     bytes += ntraces * sizeof(float);                       // receiver data
     bytes += 2 * (tiles_bound * sizeof(int) + (nrec + 1) * sizeof(Entry));
@@ -401,6 +411,7 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     } else {
         if (!rc) rc = job_alloc(b, j.allocs, &d.snap16, (size_t)m.nsnap * m.n_int, "snapshot store");
         if (!rc) rc = job_alloc(b, j.allocs, &d.snap_max, (size_t)m.nsnap, "snapshot scales");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.wave_max, max_job_tiles(m), "snapshot tile maxima");
     }
     if (rc) return rc;
     hipError_t e = hipMemcpy(tr, j.traces.data(), j.traces.size() * sizeof(float), hipMemcpyHostToDevice);
@@ -462,6 +473,8 @@ int rebuild(rtm_hip_batch* b) {
         hipError_t e = hipMemcpy(g.d_tile_job, tile_job.data(), tile_job.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e != hipSuccess) return b->hip_fail(e, "tile table upload");
     }
+    b->any_16bit = false;
+    for (Job* j : b->active) b->any_16bit |= j->model->params.snapshot_format != RTM_HIP_SNAP_F32;
     // every clock is known on the host: one step per tick since admission
     std::vector<JobDesc> descs;
     for (Job* j : b->active) {
@@ -486,6 +499,10 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks) {
             hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job, g.ntiles,
                                                       sub, 0, b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
+        }
+        if (b->any_16bit) {
+            hipError_t e = rtm_hip::launch_batch_reduce(b->d_jobs, (int)b->active.size(), sub, b->stream);
+            if (e != hipSuccess) return b->hip_fail(e, "snapshot max launch");
         }
     }
     hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);

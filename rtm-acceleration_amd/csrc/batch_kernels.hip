@@ -64,7 +64,7 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dr
     const long long prev = quant ? (it - 1) / J.store : 0;
     a.snap_out = (wide && fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
     a.snap_in = (wide && !fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
-    a.snap_max = (!wide && fwd && stored) ? reinterpret_cast<unsigned*>(J.snap_max + slot) : nullptr;
+    a.wave_max = (!wide && fwd && stored) ? J.wave_max : nullptr;
     a.quant_out = quant ? J.snap16 + prev * J.n_int : nullptr;
     a.quant_max = quant ? J.snap_max + prev : nullptr;
     a.snap16_max = (!wide && !fwd && stored) ? J.snap_max + slot : nullptr;
@@ -101,6 +101,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restri
         constexpr bool DPP = F == Family::kMarchDpp;
         if (total < J.nt) march_tile<H, TX, Dir::Forward, DPP>(a, J.coef, t, lane);
         else march_tile<H, TX, Dir::Backward, DPP>(a, J.coef, t, lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_reduce(const JobDesc* __restrict__ jobs, int sub) {
+    const JobDesc& J = jobs[blockIdx.x];
+    const int it = J.done + sub;             // forward step just done, if any
+    if (J.snap_fmt == (int)SnapFmt::F32 || it >= J.nt || (it % J.store) != 0) return;   // whole block
+    float m = 0.0f;
+    for (int i = (int)threadIdx.x; i < J.ntiles; i += (int)blockDim.x) m = fmaxf(m, J.wave_max[i]);
+    m = wave_max(m);
+    __shared__ float wm[256 / kTileZ];
+    if ((threadIdx.x & (kTileZ - 1)) == 0) wm[threadIdx.x / kTileZ] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int k = 0; k < 256 / kTileZ; ++k) r = fmaxf(r, wm[k]);
+        J.snap_max[it / J.store] = r;
     }
 }
 
@@ -178,6 +195,12 @@ hipError_t launch_stack_add(const float* image, const float* illum, int nx, int 
     const long long n = (long long)nx * nz;
     hipLaunchKernelGGL(k_stack_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, image, illum, nx, nz,
                        stack_i, stack_s, stack_nz, ix0, iz0);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reduce(JobDesc* jobs, int njobs, int sub, hipStream_t stream) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)njobs), dim3(256), 0, stream, jobs, sub);
     return hipGetLastError();
 }
 

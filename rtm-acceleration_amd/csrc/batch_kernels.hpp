@@ -37,6 +37,7 @@ struct JobDesc {
     float* snap32;               // FP32 snapshots, or
     uint16_t* snap16;            // 16-bit codes +
     float* snap_max;             // per-snapshot max|p|
+    float* wave_max;             // per-tile maxima of the latest stored step
     float* image;
     float* illum;
     const int* tile_ptr[2];      // per-tile injection tables (forward, backward)
@@ -55,6 +56,11 @@ hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc*
 // (and the same for illumination), for the job's nx x nz image.
 hipError_t launch_stack_add(const float* image, const float* illum, int nx, int nz, double* stack_i,
                             double* stack_s, int stack_nz, int ix0, int iz0, hipStream_t stream);
+
+// After tick `sub` of a run: for every job whose step just done was a stored
+// forward step with a 16-bit snapshot, snap_max[slot] = max of its per-tile
+// maxima (one workgroup per job).
+hipError_t launch_batch_reduce(JobDesc* jobs, int njobs, int sub, hipStream_t stream);
 
 // Job clocks: every job has done `ticks` more steps.
 hipError_t launch_batch_advance(JobDesc* jobs, int njobs, int ticks, hipStream_t stream);

@@ -68,6 +68,8 @@ struct rtm_hip_engine {
     float* d_snap = nullptr;         // FP32 snapshots
     uint16_t* d_snap16 = nullptr;    // FP16/BF16 snapshot codes
     float* d_snap_max = nullptr;     // per-snapshot max|p| (16-bit formats)
+    float* d_wave_max = nullptr;     // per-wave maxima of the latest stored step
+    int wave_max_cap = 0;
     size_t snap_bytes = 0;
     float* d_image = nullptr;
     float* d_illum = nullptr;
@@ -155,7 +157,7 @@ struct rtm_hip_engine {
         };
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
-        f(d_snap); f(d_snap16); f(d_snap_max); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_image); f(d_illum); f(d_traces); f(d_rec);
         f(d_stack_i); f(d_stack_s);
         for (int k = 0; k < 2; ++k) {
             f(d_tile_ptr[k]); f(d_ent[k]);
@@ -320,7 +322,7 @@ int validate_shot(rtm_hip_engine* h, const rtm_hip_shot* s, bool need_traces) {
 void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepArgs& a) {
     a.snap_fmt = h->snap_fmt;
     a.snap_out = nullptr;
-    a.snap_max = nullptr;
+    a.wave_max = nullptr;
     a.quant_out = nullptr;
     a.quant_max = nullptr;
     a.snap_in = nullptr;
@@ -335,13 +337,8 @@ void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepAr
         return;
     }
     if (dir == Dir::Forward) {
-        if (stored) a.snap_max = reinterpret_cast<unsigned*>(h->d_snap_max + slot);
-        // p^n of this launch is the field of step it-1: quantise it if stored
-        if (it >= 1 && ((it - 1) % h->store) == 0) {
-            const size_t prev = (size_t)((it - 1) / h->store);
-            a.quant_out = h->d_snap16 + prev * h->n_int;
-            a.quant_max = h->d_snap_max + prev;
-        }
+        // per-wave maxima of the new field; run_loop reduces and quantises
+        if (stored) a.wave_max = h->d_wave_max;
     } else if (stored) {
         a.snap16_in = h->d_snap16 + slot * h->n_int;
         a.snap16_max = h->d_snap_max + slot;
@@ -407,6 +404,19 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
         if (sample) h->ev_pending.push_back({k, ev, 1});
         h->launches[k] += 1;
+        // A stored 16-bit step: reduce the per-wave maxima to the snapshot's
+        // max, then quantise the new field with one thread per interior cell
+        // in snapshot order.
+        if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && (it % h->store) == 0) {
+            const size_t slot = (size_t)(it / h->store);
+            const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
+            e = rtm_hip::launch_reduce_max(h->d_wave_max, nparts, h->d_snap_max + slot, h->stream);
+            if (e == hipSuccess)
+                e = rtm_hip::launch_quantise(h->d_fld[ic], h->ldz, h->nb, h->nx, h->nz,
+                                             static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
+                                             h->d_snap16 + slot * h->n_int, h->stream);
+            if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
+        }
         // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
         const int old = ia;
         ia = ib;
@@ -416,14 +426,6 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     if (phase_ev.start) {
         (void)hipEventRecord(phase_ev.stop, h->stream);
         h->ev_pending.push_back({k, phase_ev, h->nt});
-    }
-    // a 16-bit snapshot of the last step has no following step to quantise it
-    if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && ((h->nt - 1) % h->store) == 0) {
-        const size_t slot = (size_t)((h->nt - 1) / h->store);
-        hipError_t e = rtm_hip::launch_quantise(h->d_fld[ib], h->ldz, h->nb, h->nx, h->nz,
-                                                static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
-                                                h->d_snap16 + slot * h->n_int, h->stream);
-        if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
     }
     if (dir == Dir::Backward && h->image_stack) {
         hipError_t e = rtm_hip::launch_fold_fp64(h->d_image, h->d_illum, h->d_stack_i, h->d_stack_s, h->n_int,
@@ -734,6 +736,16 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     } else {
         if (!rc) rc = h->alloc(&h->d_snap16, h->nsnap * h->n_int, "snapshot store");
         if (!rc) rc = h->alloc(&h->d_snap_max, (size_t)h->nsnap, "snapshot scales");
+        // per-wave maxima: as many as the largest launch of any step-kernel mapping
+        int parts = 0;
+        for (auto f : {rtm_hip::Family::kLdsTile, rtm_hip::Family::kMarch, rtm_hip::Family::kMarchDpp,
+                       rtm_hip::Family::kStream})
+            for (int tx : rtm_hip::tile_widths(f)) {
+                const rtm_hip::StepGeometry g = rtm_hip::step_geometry(f, h->half, tx, h->nxe, h->nze, h->ldz);
+                parts = std::max(parts, f == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles);
+            }
+        if (!rc) rc = h->alloc(&h->d_wave_max, (size_t)parts, "snapshot wave maxima");
+        h->wave_max_cap = parts;
     }
     if (h->image_stack) {
         if (!rc) rc = h->alloc(&h->d_stack_i, h->n_int, "FP64 image stack");

@@ -138,7 +138,7 @@ __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
     c.imaging = d == Dir::Backward &&
                 (wide ? a.snap_in != nullptr : (a.snap16_in != nullptr || a.pending_field != nullptr));
     c.store32 = d == Dir::Forward && wide && a.snap_out != nullptr;
-    c.store_max = d == Dir::Forward && !wide && a.snap_max != nullptr;
+    c.store_max = d == Dir::Forward && !wide && a.wave_max != nullptr;
     c.quantise = d == Dir::Forward && !wide && a.quant_out != nullptr;
     c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*gbl(a.snap16_max)) : 0;
     c.k_q = c.quantise ? snapshot_scale_exponent(*gbl(a.quant_max)) : 0;
@@ -253,10 +253,11 @@ __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc
     }
 }
 
-__device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc, float vmax, int lane) {
+// The wave's max |p^{n+1}| of a stored 16-bit step, to partial `part`.
+__device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc, float vmax, int lane, int part) {
     if (sc.store_max && !a.dry) {
         vmax = wave_max(vmax);
-        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
+        if (lane == 0) gbl(a.wave_max)[part] = vmax;
     }
 }
 
@@ -296,7 +297,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         if (!z_out || x >= a.nxe - H) continue;
         finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, e0, e1, vmax);
     }
-    finish_tile(a, sc, vmax, lane);
+    finish_tile(a, sc, vmax, lane, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,7 +371,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
             for (int r = 0; r < C; ++r) ops[r] = nops[r];
         }
     }
-    finish_tile(a, sc, vmax, lane);
+    finish_tile(a, sc, vmax, lane, t);
 }
 
 }  // namespace device

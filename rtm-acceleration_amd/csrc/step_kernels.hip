@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     }
     if (sc.store_max) {
         vmax = wave_max(vmax);
-        if (lane == 0 && vmax > 0.0f) atomicMax(a.snap_max, __float_as_uint(vmax));
+        if (lane == 0) a.wave_max[t * kWaves + wave] = vmax;
     }
 }
 
@@ -209,6 +209,22 @@ __global__ __launch_bounds__(256) void k_quantise(const float* p, int ldz, int n
     const int jx = (int)(j / (size_t)nz), jz = (int)(j % (size_t)nz);
     const int k = snapshot_scale_exponent(*maxv);
     out[j] = encode_snapshot(p[(ptrdiff_t)(jx + nb) * ldz + jz + nb], k, fmt);
+}
+
+// *out = max(parts[0..n)), one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_reduce_max(const float* parts, int n, float* out) {
+    float m = 0.0f;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) m = fmaxf(m, parts[i]);
+    m = wave_max(m);
+    __shared__ float wm[1024 / kTileZ];
+    const int w = (int)threadIdx.x / kTileZ;
+    if ((threadIdx.x & (kTileZ - 1)) == 0) wm[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int k = 0; k < (int)blockDim.x / kTileZ; ++k) r = fmaxf(r, wm[k]);
+        *out = r;
+    }
 }
 
 // FP64 cross-shot stack: stack += (double)image; image = 0.
@@ -313,6 +329,11 @@ hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs&
                        hipEvent_t start, hipEvent_t stop) {
     return dir == Dir::Forward ? launch_dir<Dir::Forward>(half, g, a, stream, start, stop)
                                : launch_dir<Dir::Backward>(half, g, a, stream, start, stop);
+}
+
+hipError_t launch_reduce_max(const float* parts, int n, float* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, stream, parts, n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_quantise(const float* p, int ldz, int nb, int nx, int nz, SnapFmt fmt, const float* max,

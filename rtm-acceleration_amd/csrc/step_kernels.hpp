@@ -16,6 +16,7 @@ constexpr int kTileZ = 64;        // one wave64 spans 64 consecutive depth cells
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kBlockThreads = kTileZ * kWaves;
 
+
 // Stencil coefficients pre-divided by the spacing (src/rtm/rtm_cpu.cpp:75-79).
 struct Coeffs {
     float c0;
@@ -61,9 +62,11 @@ struct StepArgs {
     long long    rec_stride;
     int          snap_fmt;    // SnapFmt
     // forward, stored step: FP32 -> the new field's interior into snap_out;
-    // F16/BF16 -> max|p^{n+1}| over the interior into *snap_max (float bits)
+    // F16/BF16 -> each wave's max|p^{n+1}| over its interior cells into
+    // wave_max[w] (w = the wave's tile, or tile*kWaves + wave for the LDS
+    // mapping); reduce_max_parts then gives the snapshot's max
     float*       snap_out;
-    unsigned*    snap_max;
+    float*       wave_max;
     // forward, step after a stored step (F16/BF16): quantise p^n's interior
     // into quant_out with the scale of *quant_max
     uint16_t*    quant_out;
@@ -134,6 +137,10 @@ hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs&
 
 // Quantise the interior of field p (extended layout) into a 16-bit snapshot
 // with the scale of *max (for a stored step that has no following step).
+// *out = max(parts[0..n)) (the max of a stored 16-bit snapshot from the
+// per-wave maxima of its step); one workgroup.
+hipError_t launch_reduce_max(const float* parts, int n, float* out, hipStream_t stream);
+
 hipError_t launch_quantise(const float* p, int ldz, int nb, int nx, int nz, SnapFmt fmt, const float* max,
                            uint16_t* out, hipStream_t stream);
 
