@@ -337,8 +337,14 @@ void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepAr
         return;
     }
     if (dir == Dir::Forward) {
-        // per-wave maxima of the new field; run_loop reduces and quantises
+        // per-wave maxima of the new field; run_loop reduces them after the launch
         if (stored) a.wave_max = h->d_wave_max;
+        // p^n of this launch is the field of step it-1: quantise it if stored
+        if (it >= 1 && ((it - 1) % h->store) == 0) {
+            const size_t prev = (size_t)((it - 1) / h->store);
+            a.quant_out = h->d_snap16 + prev * h->n_int;
+            a.quant_max = h->d_snap_max + prev;
+        }
     } else if (stored) {
         a.snap16_in = h->d_snap16 + slot * h->n_int;
         a.snap16_max = h->d_snap_max + slot;
@@ -405,13 +411,13 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         if (sample) h->ev_pending.push_back({k, ev, 1});
         h->launches[k] += 1;
         // A stored 16-bit step: reduce the per-wave maxima to the snapshot's
-        // max, then quantise the new field with one thread per interior cell
-        // in snapshot order.
+        // max; the next step, which reads the new field anyway as p^n, writes
+        // its codes (the last step's field is quantised by its own pass).
         if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && (it % h->store) == 0) {
             const size_t slot = (size_t)(it / h->store);
             const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
             e = rtm_hip::launch_reduce_max(h->d_wave_max, nparts, h->d_snap_max + slot, h->stream);
-            if (e == hipSuccess)
+            if (e == hipSuccess && it == h->nt - 1)
                 e = rtm_hip::launch_quantise(h->d_fld[ic], h->ldz, h->nb, h->nx, h->nz,
                                              static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
                                              h->d_snap16 + slot * h->n_int, h->stream);
