@@ -221,11 +221,15 @@ def served_workload(args):
     for rate in rates:
         reqs = serve.served_stream(specs, rate, args.jobs, seed=int(rate * 1000) + 7)
         print(f"bench: serving {len(reqs)} requests at {rate}/s", file=sys.stderr, flush=True)
-        batched = serve.serve(specs, reqs, reserve=args.serve_reserve_gib << 30)
-        plain = serve.serial_plain(specs, reqs)
-        sweep.append({"rate_per_s": rate, "batched": _round(batched), "plain_serial": _round(plain),
-                      "shots_per_s_ratio": round(batched["shots_per_s"] / plain["shots_per_s"], 3),
-                      "p99_ratio": round(batched["latency_p99_s"] / plain["latency_p99_s"], 3)})
+        batched = serve.serve(specs, reqs, reserve=args.serve_reserve_gib << 30, use_graph=not args.batch_no_graph,
+                              chunk_s=args.serve_chunk_ms * 1e-3)
+        plain = serve.serial_plain(specs, reqs) if not args.no_plain_serial else None
+        row = {"rate_per_s": rate, "batched": _round(batched)}
+        if plain:
+            row.update({"plain_serial": _round(plain),
+                        "shots_per_s_ratio": round(batched["shots_per_s"] / plain["shots_per_s"], 3),
+                        "p99_ratio": round(batched["latency_p99_s"] / plain["latency_p99_s"], 3)})
+        sweep.append(row)
     top = sweep[-1]
     return {
         "metric": METRIC, "value": round(top["batched"]["gcell_steps_per_s"], 3), "unit": "Gcell-steps/s",
@@ -330,6 +334,9 @@ def main():
     ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
     ap.add_argument("--rates", default="2,8,32,64", help="arrival rates (1/s) of the served workload")
     ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
+    ap.add_argument("--no-plain-serial", action="store_true", help="served workload without the plain baseline")
+    ap.add_argument("--serve-chunk-ms", type=float, default=10.0,
+                    help="device time the served loop runs between looking at arrivals")
     ap.add_argument("--serve-reserve-gib", type=int, default=160,
                     help="device memory the served batch engine reserves for jobs before the stream starts")
     args = ap.parse_args()
@@ -393,7 +400,8 @@ def main():
                                "767 receivers, plain per-shot engine, FP32 snapshots",
                    "engine": "hip",
                    "step_kernel": {d: {"family": ["lds-tile", "march", "march-dpp", "stream"][info["step_family"][k]],
-                                       "tile_x": info["tile_x"][k], "autotuned": bool(info["autotuned"][k])}
+                                       "tile_x": info["tile_x"][k], "autotuned": bool(info["autotuned"][k]),
+                                       "waves_per_cu": info["waves_per_cu"][k] or "no cap"}
                                    for k, d in enumerate(("forward", "backward"))},
                    "ldz": info["ldz"]},
         "shots_per_s": round(K / elapsed, 4),

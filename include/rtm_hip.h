@@ -97,6 +97,7 @@ typedef struct {
     int    step_family[2];       /* RTM_HIP_STEP_* of the forward/backward step kernel */
     int    tile_x[2];            /* their tile widths along x            */
     int    autotuned[2];         /* 1 when chosen by the autotuner       */
+    int    waves_per_cu[2];      /* resident wavefronts per CU the launch is capped at (0: no cap) */
     char   device_name[128];
 } rtm_hip_info;
 
@@ -271,6 +272,7 @@ typedef struct {
     size_t    bytes_in_use, bytes_budget;
     int       ngroups;           /* stencil half-widths with a launch per tick     */
     int       group_half[6], group_tile_x[6], group_family[6];   /* RTM_HIP_STEP_* */
+    int       group_waves_per_cu[6];                             /* 0: no cap */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

@@ -101,6 +101,8 @@ struct Job {
 struct Group {
     rtm_hip::Family family = rtm_hip::Family::kMarchDpp;
     int tile_x = 4;
+    int blocks_per_cu = 0;           // occupancy cap (0 = none) and the LDS that enforces it
+    int lds_bytes = 0;
     bool tuned = false;
     int ntiles = 0;
     int* d_tile_job = nullptr;
@@ -497,7 +499,7 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks) {
         for (auto& [half, g] : b->groups) {
             if (g.ntiles == 0) continue;
             hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job, g.ntiles,
-                                                      sub, 0, b->stream);
+                                                      sub, 0, g.lds_bytes, b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
         }
         if (b->any_16bit) {
@@ -509,7 +511,7 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Kernel family and tile width per half-width, picked by timing dry
+// Kernel family, tile width and wavefronts per CU per half-width, picked by timing dry
 // steps (no clock advance, no image or snapshot update: the next real step
 // rewrites the same p^{n+1}) on the live jobs; a candidate must beat the
 // best by 5 % to replace it.
@@ -531,47 +533,70 @@ int autotune(rtm_hip_batch* b) {
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
+    // 2 warm dry steps, then the best of 3 rounds of 4 timed dry steps
+    auto time_group = [&](int half, const Group& g) -> float {
+        float ms = 1e30f;
+        for (int round = 0; round < 3 && !rc; ++round) {
+            for (int rep = 0; rep < (round == 0 ? 6 : 4) && !rc; ++rep) {
+                if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
+                hipError_t le = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job,
+                                                           g.ntiles, 0, 1, g.lds_bytes, b->stream);
+                if (le != hipSuccess) rc = b->hip_fail(le, "autotune launch");
+            }
+            if (rc) break;
+            (void)hipEventRecord(t1, b->stream);
+            hipError_t se = hipEventSynchronize(t1);
+            if (se != hipSuccess) {
+                rc = b->hip_fail(se, "autotune");
+                break;
+            }
+            float round_ms = 0.0f;
+            (void)hipEventElapsedTime(&round_ms, t0, t1);
+            ms = std::min(ms, round_ms);
+        }
+        return ms;
+    };
     for (auto& [half, g] : b->groups) {
         if (rc || g.ntiles == 0 || g.tuned) continue;
+        // stage 1: mapping and tile width at full occupancy
         float best = 1e30f;
-        int best_tx = g.tile_x;
-        rtm_hip::Family best_fam = g.family;
+        Group pick = g;
         for (auto fam : {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch}) {
             for (int tx : rtm_hip::tile_widths(fam)) {
                 if (rc) break;
                 g.tile_x = tx;
                 g.family = fam;
+                g.lds_bytes = 0;
+                g.blocks_per_cu = 0;
                 if ((rc = rebuild(b))) break;
-                // 2 warm steps, then the best of 3 rounds of 4 timed steps
-                float ms = 1e30f;
-                for (int round = 0; round < 3 && !rc; ++round) {
-                    for (int rep = 0; rep < (round == 0 ? 6 : 4) && !rc; ++rep) {
-                        if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
-                        e = rtm_hip::launch_batch_step(half, fam, tx, b->d_jobs, g.d_tile_job, g.ntiles, 0, 1,
-                                                       b->stream);
-                        if (e != hipSuccess) rc = b->hip_fail(e, "autotune launch");
-                    }
-                    if (rc) break;
-                    (void)hipEventRecord(t1, b->stream);
-                    e = hipEventSynchronize(t1);
-                    if (e != hipSuccess) {
-                        rc = b->hip_fail(e, "autotune");
-                        break;
-                    }
-                    float round_ms = 0.0f;
-                    (void)hipEventElapsedTime(&round_ms, t0, t1);
-                    ms = std::min(ms, round_ms);
-                }
+                const float ms = time_group(half, g);
                 if (rc) break;
                 if (ms < 0.95f * best) {
                     best = ms;
-                    best_tx = tx;
-                    best_fam = fam;
+                    pick = g;
                 }
             }
         }
-        g.tile_x = best_tx;
-        g.family = best_fam;
+        // stage 2: wavefronts per CU for the winner
+        g.tile_x = pick.tile_x;
+        g.family = pick.family;
+        if (!rc && (rc = rebuild(b)) == RTM_HIP_OK) {
+            for (int cap : {4, 2}) {
+                g.blocks_per_cu = cap;
+                g.lds_bytes = rtm_hip::occupancy_lds_bytes(b->device, cap);
+                if (g.lds_bytes == 0) continue;
+                const float ms = time_group(half, g);
+                if (rc) break;
+                if (ms < 0.95f * best) {
+                    best = ms;
+                    pick = g;
+                }
+            }
+        }
+        g.tile_x = pick.tile_x;
+        g.family = pick.family;
+        g.blocks_per_cu = pick.blocks_per_cu;
+        g.lds_bytes = pick.lds_bytes;
         g.tuned = true;
         b->dirty = true;
     }
@@ -1002,6 +1027,7 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
         out->group_half[k] = half;
         out->group_tile_x[k] = g.tile_x;
         out->group_family[k] = (int)g.family;
+        out->group_waves_per_cu[k] = g.blocks_per_cu * rtm_hip::kWaves;
         ++k;
     }
     out->ngroups = k;

@@ -139,11 +139,11 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 
 template <int H>
 hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* tile_job, int ntiles, int sub,
-                    int dry, hipStream_t s) {
+                    int dry, size_t lds, hipStream_t s) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
 #define RTM_BATCH(TX, F) \
-    hipLaunchKernelGGL((k_batch<H, TX, F>), grid, block, 0, s, jobs, tile_job, ntiles, sub, dry)
+    hipLaunchKernelGGL((k_batch<H, TX, F>), grid, block, lds, s, jobs, tile_job, ntiles, sub, dry)
     if (family == Family::kStream) {
         switch (tile_x) {
             case 8: RTM_BATCH(8, Family::kStream); break;
@@ -177,15 +177,16 @@ hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* t
 }  // namespace
 
 hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int sub, int dry, hipStream_t stream) {
+                             int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream) {
+    const size_t lds = (size_t)lds_bytes;
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
-        case 2: return launch_h<2>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
-        case 3: return launch_h<3>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
-        case 4: return launch_h<4>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
-        case 5: return launch_h<5>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
-        case 6: return launch_h<6>(family, tile_x, jobs, tile_job, ntiles, sub, dry, stream);
+        case 1: return launch_h<1>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 2: return launch_h<2>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 3: return launch_h<3>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 4: return launch_h<4>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 5: return launch_h<5>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 6: return launch_h<6>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
         default: return hipErrorInvalidValue;
     }
 }

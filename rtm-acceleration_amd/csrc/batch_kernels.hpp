@@ -48,9 +48,10 @@ struct JobDesc {
 // the job of global tile g, and job j computes its step jobs[j].done + sub;
 // family kMarch, kMarchDpp or kStream with the tile widths of the per-shot
 // kernels.  dry != 0 writes only p^{n+1}: no image, illumination or snapshot
-// update (used to time candidate kernels on live state).
+// update (used to time candidate kernels on live state).  lds_bytes of
+// dynamic LDS per workgroup cap the workgroups resident on a CU.
 hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int sub, int dry, hipStream_t stream);
+                             int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.

@@ -490,8 +490,7 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     // per candidate: 4 warm launches, then the best of 3 rounds of 16 timed
     // launches (the minimum is the least noisy estimate of a kernel's time)
     constexpr int kWarm = 4, kRounds = 3, kTimed = 16;
-    for (const Candidate& c : cands) {
-        const rtm_hip::StepGeometry g = rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
+    auto time_geometry = [&](const rtm_hip::StepGeometry& g) -> float {
         a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
         float cand_ms = 1e30f;
         int n = 0;
@@ -518,12 +517,35 @@ int autotune(rtm_hip_engine* h, Dir dir) {
             (void)hipEventElapsedTime(&ms, ev.start, ev.stop);
             cand_ms = std::min(cand_ms, ms);
         }
+        return cand_ms;
+    };
+    // stage 1: mapping and tile width at full occupancy; candidates are
+    // tried in order of preference and a later one must win by 5 % so that
+    // timing noise does not flip the choice
+    for (const Candidate& c : cands) {
+        const rtm_hip::StepGeometry g = rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
+        const float ms = time_geometry(g);
         if (rc) break;
-        // candidates are tried in order of preference; a later one must win
-        // by 5 % so that timing noise does not flip the choice
-        if (cand_ms < 0.95f * best_ms) {
-            best_ms = cand_ms;
+        if (ms < 0.95f * best_ms) {
+            best_ms = ms;
             best = g;
+        }
+    }
+    // stage 2: wavefronts per CU for the winner (fewer resident workgroups
+    // leave more L2 to each)
+    if (!rc && best.family != rtm_hip::Family::kLdsTile) {
+        const rtm_hip::StepGeometry base = best;
+        for (int cap : {4, 2}) {
+            rtm_hip::StepGeometry g = base;
+            g.blocks_per_cu = cap;
+            g.lds_bytes = rtm_hip::occupancy_lds_bytes(h->device, cap);
+            if (g.lds_bytes == 0) continue;
+            const float ms = time_geometry(g);
+            if (rc) break;
+            if (ms < 0.95f * best_ms) {
+                best_ms = ms;
+                best = g;
+            }
         }
     }
     h->ev_pool.push_back(ev);
@@ -803,6 +825,7 @@ int rtm_hip_get_info(const rtm_hip_engine* h, rtm_hip_info* out) {
             out->tile_x[k] = g.tile_x;
             out->step_family[k] = static_cast<int>(g.family);
             out->autotuned[k] = h->tuned[k] && h->family_user < 0 && h->tile_x_user == 0;
+            out->waves_per_cu[k] = g.blocks_per_cu * rtm_hip::kWaves;
         }
     }
     hipDeviceProp_t prop;

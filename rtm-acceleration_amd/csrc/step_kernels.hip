@@ -244,12 +244,13 @@ __global__ __launch_bounds__(256) void k_fold(float* image, float* illum, double
 // events carry the kernel's own start and end timestamps.
 #define RTM_LAUNCH(KERNEL)                                                                   \
     do {                                                                                     \
-        if (t0) hipExtLaunchKernelGGL((KERNEL), grid, block, 0, s, t0, t1, 0, a);            \
-        else hipLaunchKernelGGL((KERNEL), grid, block, 0, s, a);                             \
+        if (t0) hipExtLaunchKernelGGL((KERNEL), grid, block, lds, s, t0, t1, 0, a);          \
+        else hipLaunchKernelGGL((KERNEL), grid, block, lds, s, a);                           \
     } while (0)
 
 template <int H, Dir D>
 hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    const size_t lds = g.family == Family::kLdsTile ? 0 : (size_t)g.lds_bytes;
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)g.blocks);
     if (g.family == Family::kLdsTile) {
@@ -301,6 +302,16 @@ hipError_t launch_dir(int half, const StepGeometry& g, const StepArgs& a, hipStr
 }
 
 }  // namespace
+
+int occupancy_lds_bytes(int device, int blocks_per_cu) {
+    if (blocks_per_cu <= 0) return 0;
+    int per_cu = 0, per_block = 0;
+    if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return 0;
+    const int lds = per_cu / blocks_per_cu;     // blocks_per_cu fit, one more does not
+    return lds <= per_block ? lds : 0;
+}
 
 StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz) {
     StepGeometry g;

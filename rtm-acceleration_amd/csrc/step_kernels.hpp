@@ -111,7 +111,16 @@ struct StepGeometry {
     int z_origin = 0;    // first depth cell of tile 0
     int ntz = 0, ntx = 0, ntiles = 0;
     int blocks = 0;      // workgroups launched
+    // occupancy cap: workgroups per CU (0 = none), enforced by reserving
+    // lds_bytes of dynamic LDS per workgroup (the march kernels use none)
+    int blocks_per_cu = 0;
+    int lds_bytes = 0;
 };
+
+// Dynamic LDS per workgroup that lets at most `blocks_per_cu` workgroups of a
+// kernel without LDS reside on one CU of `device`; 0 when no cap is asked or
+// the device cannot give a workgroup that much.
+int occupancy_lds_bytes(int device, int blocks_per_cu);
 
 StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz);
 

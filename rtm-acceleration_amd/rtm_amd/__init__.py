@@ -98,7 +98,7 @@ class HipEngine:
         i = _capi.Info()
         self._check(self.lib.rtm_hip_get_info(self.h, C.byref(i)))
         out = {n: getattr(i, n) for n, _ in _capi.Info._fields_}
-        for n in ("step_family", "tile_x", "autotuned"):
+        for n in ("step_family", "tile_x", "autotuned", "waves_per_cu"):
             out[n] = list(out[n])
         out["device_name"] = i.device_name.decode()
         return out
@@ -322,7 +322,8 @@ class BatchEngine:
         k = s.ngroups
         names = ["lds-tile", "march", "march-dpp", "stream"]
         out["groups"] = [{"half": s.group_half[i], "tile_x": s.group_tile_x[i],
-                          "family": names[s.group_family[i]]} for i in range(k)]
-        for n in ("group_half", "group_tile_x", "group_family"):
+                          "family": names[s.group_family[i]], "waves_per_cu": s.group_waves_per_cu[i]}
+                         for i in range(k)]
+        for n in ("group_half", "group_tile_x", "group_family", "group_waves_per_cu"):
             del out[n]
         return out

@@ -49,7 +49,7 @@ class Info(C.Structure):
                 ("store_interval", C.c_int), ("num_snapshots", C.c_int),
                 ("snapshot_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
                 ("snapshot_format", C.c_int), ("image_stack", C.c_int), ("step_family", C.c_int * 2), ("tile_x", C.c_int * 2),
-                ("autotuned", C.c_int * 2),
+                ("autotuned", C.c_int * 2), ("waves_per_cu", C.c_int * 2),
                 ("device_name", C.c_char * 128)]
 
 
@@ -75,7 +75,8 @@ class BatchStats(C.Structure):
                 ("admit_seconds", C.c_double), ("tune_seconds", C.c_double), ("retire_seconds", C.c_double),
                 ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
-                ("group_tile_x", C.c_int * 6), ("group_family", C.c_int * 6)]
+                ("group_tile_x", C.c_int * 6), ("group_family", C.c_int * 6),
+                ("group_waves_per_cu", C.c_int * 6)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.
