@@ -268,7 +268,7 @@ size_t max_job_tiles(const Model& m) {
 
 size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
     const size_t snap_elem = m.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
-    const size_t tiles_bound = (size_t)(m.nxe / 4 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 4, strip >= 52
+    const size_t tiles_bound = (size_t)(m.nxe / 2 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 2, strip >= 52
     size_t bytes = 0;
     bytes += 3 * m.n_fld * sizeof(float);                   // three wavefields
     bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store

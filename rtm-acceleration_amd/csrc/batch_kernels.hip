@@ -153,6 +153,8 @@ hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* t
         }
     } else if (family == Family::kMarchDpp) {
         switch (tile_x) {
+            case 2: RTM_BATCH(2, Family::kMarchDpp); break;
+            case 3: RTM_BATCH(3, Family::kMarchDpp); break;
             case 4: RTM_BATCH(4, Family::kMarchDpp); break;
             case 5: RTM_BATCH(5, Family::kMarchDpp); break;
             case 6: RTM_BATCH(6, Family::kMarchDpp); break;

@@ -972,7 +972,7 @@ int rtm_hip_set_step_kernel(rtm_hip_engine* h, int family, int tile_x) {
         }
         if (!ok)
             return h->fail(RTM_HIP_ERR_ARG, "tile width %d is not available for step kernel family %d "
-                           "(march-dpp: 4, 5, 6, 8, 16; march, lds-tile: 4, 8, 16; stream: 8, 16, 32)",
+                           "(march-dpp: 2, 3, 4, 5, 6, 8, 16; march, lds-tile: 4, 8, 16; stream: 8, 16, 32)",
                            tile_x, family);
     }
     h->family_user = family;

@@ -276,6 +276,8 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
         }
     } else {
         switch (g.tile_x) {
+            case 2:  RTM_LAUNCH((k_march<H, 2, D, true>)); break;
+            case 3:  RTM_LAUNCH((k_march<H, 3, D, true>)); break;
             case 4:  RTM_LAUNCH((k_march<H, 4, D, true>)); break;
             case 5:  RTM_LAUNCH((k_march<H, 5, D, true>)); break;
             case 6:  RTM_LAUNCH((k_march<H, 6, D, true>)); break;

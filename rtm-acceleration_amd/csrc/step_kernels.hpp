@@ -129,7 +129,7 @@ StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze
 // chip's resident-wave capacity instead of just over it.
 inline std::vector<int> tile_widths(Family f) {
     switch (f) {
-        case Family::kMarchDpp: return {4, 5, 6, 8, 16};
+        case Family::kMarchDpp: return {4, 3, 2, 5, 6, 8, 16};
         case Family::kStream: return {8, 16, 32};
         default: return {4, 8, 16};
     }

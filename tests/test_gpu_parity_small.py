@@ -35,7 +35,7 @@ def test_model_shot_bit_identical(hip_engine_cls, order):
             assert bits_equal(ref, got), f"max diff {np.abs(ref - got).max()}"
 
 
-KERNELS = [(-1, 0), (0, 4), (0, 8), (0, 16), (1, 4), (1, 8), (1, 16), (2, 4), (2, 5), (2, 6), (2, 8), (2, 16),
+KERNELS = [(-1, 0), (0, 4), (0, 8), (0, 16), (1, 4), (1, 8), (1, 16), (2, 2), (2, 3), (2, 4), (2, 5), (2, 6), (2, 8), (2, 16),
            (3, 8), (3, 16), (3, 32)]  # (family, tile_x)
 
 
