@@ -315,7 +315,7 @@ def main():
                     help="step-kernel family: 2 = DPP march (default; the fastest on this grid in the full "
                          "loop), -1 = autotune, 0 LDS tile, 1 march, 3 stream")
     ap.add_argument("--tile", type=int, default=3,
-                    help="step-kernel tile width (default 3: 4 % faster than 4 in the full loop, within the "
+                    help="step-kernel tile width (default 3: 4 %% faster than 4 in the full loop, within the "
                          "autotuner's 5 %% margin on its synthetic launches; 0 = built-in)")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batched comparison with direct launches instead of hipGraph replay")
