@@ -511,10 +511,10 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Kernel family, tile width and wavefronts per CU per half-width, picked by timing dry
-// steps (no clock advance, no image or snapshot update: the next real step
-// rewrites the same p^{n+1}) on the live jobs; a candidate must beat the
-// best by 5 % to replace it.
+// Kernel family, tile width and wavefronts per CU per half-width, picked by
+// timing dry steps (no clock advance, no image or snapshot update: the next
+// real step rewrites the same p^{n+1}) on the live jobs; the launches are
+// long, so a candidate replaces the best when it is 2 % faster.
 int autotune(rtm_hip_batch* b) {
     if (b->cfg.kernel_family != 0) {
         for (auto& [half, g] : b->groups) {
@@ -533,11 +533,11 @@ int autotune(rtm_hip_batch* b) {
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
-    // 2 warm dry steps, then the best of 3 rounds of 4 timed dry steps
+    // 2 warm dry steps, then the best of 3 rounds of 8 timed dry steps
     auto time_group = [&](int half, const Group& g) -> float {
         float ms = 1e30f;
         for (int round = 0; round < 3 && !rc; ++round) {
-            for (int rep = 0; rep < (round == 0 ? 6 : 4) && !rc; ++rep) {
+            for (int rep = 0; rep < (round == 0 ? 10 : 8) && !rc; ++rep) {
                 if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
                 hipError_t le = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job,
                                                            g.ntiles, 0, 1, g.lds_bytes, b->stream);
@@ -562,7 +562,13 @@ int autotune(rtm_hip_batch* b) {
         float best = 1e30f;
         Group pick = g;
         for (auto fam : {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch}) {
-            for (int tx : rtm_hip::tile_widths(fam)) {
+            // widest first: with many jobs per launch there are waves to spare
+            // and wide tiles re-read less halo (a narrower one must win by 2 %)
+            std::vector<int> widths = rtm_hip::tile_widths(fam);
+            std::sort(widths.begin(), widths.end(), [](int x, int y) {
+                return (x == 16) != (y == 16) ? y == 16 : x > y;      // 16 last: it spills registers
+            });
+            for (int tx : widths) {
                 if (rc) break;
                 g.tile_x = tx;
                 g.family = fam;
@@ -571,7 +577,7 @@ int autotune(rtm_hip_batch* b) {
                 if ((rc = rebuild(b))) break;
                 const float ms = time_group(half, g);
                 if (rc) break;
-                if (ms < 0.95f * best) {
+                if (ms < 0.98f * best) {
                     best = ms;
                     pick = g;
                 }
@@ -587,7 +593,7 @@ int autotune(rtm_hip_batch* b) {
                 if (g.lds_bytes == 0) continue;
                 const float ms = time_group(half, g);
                 if (rc) break;
-                if (ms < 0.95f * best) {
+                if (ms < 0.98f * best) {
                     best = ms;
                     pick = g;
                 }
