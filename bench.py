@@ -118,12 +118,13 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
                       f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
 
 
-def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0, use_graph=True):
+def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0, use_graph=True,
+                       kernel_family=0, tile_x=0):
     """The batch engine on the same shots: one resident model, every shot
     submitted at once and advanced together, one launch per time step
     (hipGraph replay).  Returns throughput and per-shot latency."""
     import rtm_amd
-    with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+    with rtm_amd.BatchEngine(0, use_graph=use_graph, kernel_family=kernel_family, tile_x=tile_x) as b:
         b.reserve()                               # job memory placed once, outside the timed region
         mid = b.add_model(model, params, nt, dt, snapshot_format)
         b.submit_shot(mid, warm_shot)             # warm-up: step-kernel autotune, graph capture
@@ -317,6 +318,9 @@ def main():
     ap.add_argument("--tile", type=int, default=3,
                     help="step-kernel tile width (default 3: 4 %% faster than 4 in the full loop, within the "
                          "autotuner's 5 %% margin on its synthetic launches; 0 = built-in)")
+    ap.add_argument("--batch-family", type=int, default=0,
+                    help="batch step kernel (0 = autotuned, 1 march, 2 DPP march, 3 stream)")
+    ap.add_argument("--batch-tile", type=int, default=0, help="batch step kernel tile width (0 = default)")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batched comparison with direct launches instead of hipGraph replay")
     ap.add_argument("--sample-every", type=int, default=-1,
@@ -422,7 +426,8 @@ def main():
     }
     if not args.no_batched:
         print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
-        bt = batched_same_shots(model, params, nt, dt, shots[W:W + NB], shots[0], use_graph=not args.batch_no_graph)
+        bt = batched_same_shots(model, params, nt, dt, shots[W:W + NB], shots[0], use_graph=not args.batch_no_graph,
+                                kernel_family=args.batch_family, tile_x=args.batch_tile)
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
     if not args.no_cpu:
