@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--batch-family", type=int, default=0,
                     help="batch step kernel (0 = autotuned, 1 march, 2 DPP march, 3 stream)")
     ap.add_argument("--batch-tile", type=int, default=0, help="batch step kernel tile width (0 = default)")
+    ap.add_argument("--batch-snapshots", choices=("f32", "f16", "bf16"), default="f32",
+                    help="snapshot format of the batched comparison's jobs")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batched comparison with direct launches instead of hipGraph replay")
     ap.add_argument("--sample-every", type=int, default=-1,
@@ -426,8 +428,10 @@ def main():
     }
     if not args.no_batched:
         print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
-        bt = batched_same_shots(model, params, nt, dt, shots[W:W + NB], shots[0], use_graph=not args.batch_no_graph,
-                                kernel_family=args.batch_family, tile_x=args.batch_tile)
+        bt = batched_same_shots(model, params, nt, dt, shots[W:W + NB], shots[0],
+                                snapshot_format=("f32", "f16", "bf16").index(args.batch_snapshots),
+                                use_graph=not args.batch_no_graph, kernel_family=args.batch_family,
+                                tile_x=args.batch_tile)
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
     if not args.no_cpu:

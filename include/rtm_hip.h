@@ -236,7 +236,7 @@ typedef struct {
     size_t memory_budget;        /* device bytes for jobs; 0 = 90% of free memory    */
     int    max_active;           /* cap on jobs on the device at once; 0 = no cap    */
     int    use_graph;            /* 1: capture one tick in a hipGraph and replay it  */
-    int    kernel_family;        /* 0: autotuned per stencil order; else RTM_HIP_STEP_MARCH,
+    int    kernel_family;        /* 0: autotuned per stencil order and load; else RTM_HIP_STEP_MARCH,
                                     RTM_HIP_STEP_MARCH_DPP or RTM_HIP_STEP_STREAM      */
     int    tile_x;               /* with kernel_family: its tile width (0: default)  */
 } rtm_hip_batch_config;
@@ -270,9 +270,11 @@ typedef struct {
     double    busy_latency_s;    /* sum of submit-to-done times of finished jobs   */
     int       models_resident;   /* registered models (rtm_hip_batch_add_model)     */
     size_t    bytes_in_use, bytes_budget;
-    int       ngroups;           /* stencil half-widths with a launch per tick     */
-    int       group_half[6], group_tile_x[6], group_family[6];   /* RTM_HIP_STEP_* */
-    int       group_waves_per_cu[6];                             /* 0: no cap */
+    int       ngroups;           /* launch groups: (half-width, snapshot width)    */
+    int       group_half[12], group_tile_x[12], group_family[12]; /* RTM_HIP_STEP_* */
+    int       group_waves_per_cu[12];                             /* 0: no cap */
+    int       group_snap16[12];  /* 1: the group's jobs store 16-bit snapshots     */
+    int       tunes;             /* autotune searches (one per group and load class) */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

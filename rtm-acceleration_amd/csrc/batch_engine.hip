@@ -23,8 +23,8 @@
 //
 // Launches.  Jobs are grouped by stencil half-width; each group has one
 // launch per tick over the concatenation of its jobs' tiles, with the tile
-// width and cross-lane method picked once per half-width by timing dry runs
-// on the live jobs.  A tick (the group launches plus the clock update), and a
+// width and cross-lane method picked per half-width and load class (octave of
+// the group's extended cells) by timing dry runs on the live jobs.  A tick (the group launches plus the clock update), and a
 // run of 32 ticks (32 sets of group launches, one clock update), are captured
 // in hipGraphs and replayed until the job set changes, so one host call
 // advances every job 32 steps.
@@ -98,16 +98,34 @@ struct Job {
     std::vector<float> image, illum;
 };
 
-struct Group {
+// Kernel choice of a half-width group.
+struct Tuning {
     rtm_hip::Family family = rtm_hip::Family::kMarchDpp;
     int tile_x = 4;
     int blocks_per_cu = 0;           // occupancy cap (0 = none) and the LDS that enforces it
     int lds_bytes = 0;
+};
+
+// The best kernel depends on how much work one launch holds: a lone small
+// grid wants narrow tiles (waves to fill 256 CUs), many large grids want wide
+// ones (less halo re-read).  Loads are classed by octave of the group's
+// extended cells, and each class is tuned once, on first sight.
+struct Group : Tuning {
     bool tuned = false;
+    int load_class = -1;             // class of the job set the tables were built for
+    std::map<int, Tuning> by_load;   // tuned choice per load class
     int ntiles = 0;
     int* d_tile_job = nullptr;
     size_t cap = 0;
 };
+
+int load_class_of(double ext_cells) { return (int)std::floor(std::log2(std::max(ext_cells, 1.0))); }
+
+// Jobs share a launch when they have the same stencil half-width and
+// snapshot width (the FP32 and 16-bit kernels are separate builds).
+int group_key(const Model& m) { return 2 * m.half + (m.params.snapshot_format != RTM_HIP_SNAP_F32 ? 1 : 0); }
+int key_half(int key) { return key / 2; }
+bool key_snap16(int key) { return (key & 1) != 0; }
 
 }  // namespace
 
@@ -125,8 +143,7 @@ struct rtm_hip_batch {
     std::vector<Job*> active;
     JobDesc* d_jobs = nullptr;
     size_t jobs_cap = 0;
-    std::map<int, Group> groups;                     // by stencil half-width
-    bool any_16bit = false;                          // an active job stores 16-bit snapshots
+    std::map<int, Group> groups;                     // by group_key (half-width, snapshot width)
     bool dirty = true;
     // captured ticks: graph[0] = one tick, graph[1] = kGraphTicks ticks
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -272,7 +289,8 @@ size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
     size_t bytes = 0;
     bytes += 3 * m.n_fld * sizeof(float);                   // three wavefields
     bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store
-    bytes += ((size_t)m.nsnap + tiles_bound) * sizeof(float);   // snapshot scales, per-tile maxima
+    if (snap_elem == 2)                                     // spare word, scales and their bookkeeping
+        bytes += ((size_t)m.nsnap + 2 * tiles_bound + 2 * rtm_hip::kReduceWaves + 5) * sizeof(float);
     bytes += 2 * m.n_int * sizeof(float);                   // image, illumination
     bytes += ntraces * sizeof(float);                       // receiver data
     bytes += 2 * (tiles_bound * sizeof(int) + (nrec + 1) * sizeof(Entry));
@@ -411,9 +429,13 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     if (d.snap_fmt == RTM_HIP_SNAP_F32) {
         if (!rc) rc = job_alloc(b, j.allocs, &d.snap32, (size_t)m.nsnap * m.n_int, "snapshot store");
     } else {
-        if (!rc) rc = job_alloc(b, j.allocs, &d.snap16, (size_t)m.nsnap * m.n_int, "snapshot store");
+        // one spare word: imaging reads the aligned 32-bit word holding a code
+        if (!rc) rc = job_alloc(b, j.allocs, &d.snap16, (size_t)m.nsnap * m.n_int + 2, "snapshot store");
         if (!rc) rc = job_alloc(b, j.allocs, &d.snap_max, (size_t)m.nsnap, "snapshot scales");
-        if (!rc) rc = job_alloc(b, j.allocs, &d.wave_max, max_job_tiles(m), "snapshot tile maxima");
+        d.parts_cap = (int)max_job_tiles(m);
+        if (!rc) rc = job_alloc(b, j.allocs, &d.wave_max, 2 * (size_t)d.parts_cap, "snapshot tile maxima");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.slice_max, 2 * (size_t)rtm_hip::kReduceWaves, "snapshot slice maxima");
+        if (!rc) rc = job_alloc(b, j.allocs, &d.part_counts, 4, "snapshot maxima counts");
     }
     if (rc) return rc;
     hipError_t e = hipMemcpy(tr, j.traces.data(), j.traces.size() * sizeof(float), hipMemcpyHostToDevice);
@@ -421,9 +443,11 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     if (e == hipSuccess) e = hipMemset(ill, 0, m.n_int * sizeof(float));
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(d.fld[k], 0, m.n_fld * sizeof(float));
     if (e == hipSuccess && d.snap_max) e = hipMemset(d.snap_max, 0, (size_t)m.nsnap * sizeof(float));
+    if (e == hipSuccess && d.part_counts) e = hipMemset(d.part_counts, 0, 4 * sizeof(int));
     if (e != hipSuccess) return b->hip_fail(e, "job upload");
     d.traces = tr; d.image = img; d.illum = ill;
-    j.steps_left = 2 * m.nt;
+    d.steps = 2 * m.nt + (d.snap_fmt == RTM_HIP_SNAP_F32 ? 0 : 1);     // + the bookkeeping tick
+    j.steps_left = d.steps;
     j.traces.clear();
     j.traces.shrink_to_fit();
     return RTM_HIP_OK;
@@ -447,12 +471,23 @@ int rebuild(rtm_hip_batch* b) {
             return b->hip_fail(e, "job descriptors");
         }
     }
-    std::map<int, std::vector<int>> by_half;
-    for (size_t k = 0; k < b->active.size(); ++k) by_half[b->active[k]->model->half].push_back((int)k);
-    for (auto& [half, g] : b->groups)
-        if (!by_half.count(half)) g.ntiles = 0;
-    for (auto& [half, idx] : by_half) {
-        Group& g = b->groups[half];
+    std::map<int, std::vector<int>> by_key;
+    for (size_t k = 0; k < b->active.size(); ++k) by_key[group_key(*b->active[k]->model)].push_back((int)k);
+    for (auto& [key, g] : b->groups)
+        if (!by_key.count(key)) g.ntiles = 0;
+    for (auto& [key, idx] : by_key) {
+        Group& g = b->groups[key];
+        if (b->cfg.kernel_family == 0) {
+            double cells = 0;
+            for (int k : idx) cells += (double)b->active[k]->model->nxe * b->active[k]->model->nze;
+            const int cls = load_class_of(cells);
+            if (cls != g.load_class) {
+                g.load_class = cls;
+                auto it = g.by_load.find(cls);
+                if (it != g.by_load.end()) static_cast<Tuning&>(g) = it->second;
+                g.tuned = it != g.by_load.end();
+            }
+        }
         std::vector<int> tile_job;
         for (int k : idx) {
             Job& j = *b->active[k];
@@ -475,12 +510,10 @@ int rebuild(rtm_hip_batch* b) {
         hipError_t e = hipMemcpy(g.d_tile_job, tile_job.data(), tile_job.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e != hipSuccess) return b->hip_fail(e, "tile table upload");
     }
-    b->any_16bit = false;
-    for (Job* j : b->active) b->any_16bit |= j->model->params.snapshot_format != RTM_HIP_SNAP_F32;
     // every clock is known on the host: one step per tick since admission
     std::vector<JobDesc> descs;
     for (Job* j : b->active) {
-        j->desc.done = 2 * j->model->nt - j->steps_left;
+        j->desc.done = j->desc.steps - j->steps_left;
         descs.push_back(j->desc);
     }
     if (!descs.empty()) {
@@ -496,28 +529,24 @@ int rebuild(rtm_hip_batch* b) {
 // update for the run.
 int enqueue_ticks(rtm_hip_batch* b, int ticks) {
     for (int sub = 0; sub < ticks; ++sub) {
-        for (auto& [half, g] : b->groups) {
+        for (auto& [key, g] : b->groups) {
             if (g.ntiles == 0) continue;
-            hipError_t e = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job, g.ntiles,
-                                                      sub, 0, g.lds_bytes, b->stream);
+            hipError_t e = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key), b->d_jobs,
+                                                      g.d_tile_job, g.ntiles, sub, 0, g.lds_bytes, b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
-        }
-        if (b->any_16bit) {
-            hipError_t e = rtm_hip::launch_batch_reduce(b->d_jobs, (int)b->active.size(), sub, b->stream);
-            if (e != hipSuccess) return b->hip_fail(e, "snapshot max launch");
         }
     }
     hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Kernel family, tile width and wavefronts per CU per half-width, picked by
+// Kernel family, tile width and wavefronts per CU per group, picked by
 // timing dry steps (no clock advance, no image or snapshot update: the next
 // real step rewrites the same p^{n+1}) on the live jobs; the launches are
 // long, so a candidate replaces the best when it is 2 % faster.
 int autotune(rtm_hip_batch* b) {
     if (b->cfg.kernel_family != 0) {
-        for (auto& [half, g] : b->groups) {
+        for (auto& [key, g] : b->groups) {
             if (g.tuned) continue;
             g.family = static_cast<rtm_hip::Family>(b->cfg.kernel_family);
             g.tile_x = b->cfg.tile_x ? b->cfg.tile_x : (g.family == rtm_hip::Family::kStream ? 16 : 8);
@@ -527,20 +556,21 @@ int autotune(rtm_hip_batch* b) {
         return RTM_HIP_OK;
     }
     bool any = false;
-    for (auto& [half, g] : b->groups) any |= g.ntiles > 0 && !g.tuned;
+    for (auto& [key, g] : b->groups) any |= g.ntiles > 0 && !g.tuned;
     if (!any) return RTM_HIP_OK;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
-    // 2 warm dry steps, then the best of 3 rounds of 8 timed dry steps
-    auto time_group = [&](int half, const Group& g) -> float {
+    // 2 warm dry steps, then the best of `rounds` rounds of `reps` timed dry steps
+    auto time_group = [&](int key, const Group& g, int rounds, int reps) -> float {
         float ms = 1e30f;
-        for (int round = 0; round < 3 && !rc; ++round) {
-            for (int rep = 0; rep < (round == 0 ? 10 : 8) && !rc; ++rep) {
+        for (int round = 0; round < rounds && !rc; ++round) {
+            for (int rep = 0; rep < (round == 0 ? reps + 2 : reps) && !rc; ++rep) {
                 if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
-                hipError_t le = rtm_hip::launch_batch_step(half, g.family, g.tile_x, b->d_jobs, g.d_tile_job,
-                                                           g.ntiles, 0, 1, g.lds_bytes, b->stream);
+                hipError_t le = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key),
+                                                           b->d_jobs, g.d_tile_job, g.ntiles, 0, 1, g.lds_bytes,
+                                                           b->stream);
                 if (le != hipSuccess) rc = b->hip_fail(le, "autotune launch");
             }
             if (rc) break;
@@ -556,12 +586,20 @@ int autotune(rtm_hip_batch* b) {
         }
         return ms;
     };
-    for (auto& [half, g] : b->groups) {
+    for (auto& [key, g] : b->groups) {
         if (rc || g.ntiles == 0 || g.tuned) continue;
+        // The first search of a group tries every mapping and occupancy cap
+        // (3 rounds of 8 steps); a new load class of a group already tuned
+        // holds jobs mid-stream, so it only re-picks the DPP march width
+        // (2 rounds of 4 steps), which is where the classes differ.
+        const bool full = g.by_load.empty();
+        const int rounds = full ? 3 : 2, reps = full ? 8 : 4;
         // stage 1: mapping and tile width at full occupancy
         float best = 1e30f;
-        Group pick = g;
-        for (auto fam : {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch}) {
+        Tuning pick = g;
+        std::vector<rtm_hip::Family> families{rtm_hip::Family::kMarchDpp};
+        if (full) families = {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch};
+        for (auto fam : families) {
             // widest first: with many jobs per launch there are waves to spare
             // and wide tiles re-read less halo (a narrower one must win by 2 %)
             std::vector<int> widths = rtm_hip::tile_widths(fam);
@@ -575,7 +613,7 @@ int autotune(rtm_hip_batch* b) {
                 g.lds_bytes = 0;
                 g.blocks_per_cu = 0;
                 if ((rc = rebuild(b))) break;
-                const float ms = time_group(half, g);
+                const float ms = time_group(key, g, rounds, reps);
                 if (rc) break;
                 if (ms < 0.98f * best) {
                     best = ms;
@@ -586,12 +624,12 @@ int autotune(rtm_hip_batch* b) {
         // stage 2: wavefronts per CU for the winner
         g.tile_x = pick.tile_x;
         g.family = pick.family;
-        if (!rc && (rc = rebuild(b)) == RTM_HIP_OK) {
+        if (full && !rc && (rc = rebuild(b)) == RTM_HIP_OK) {
             for (int cap : {4, 2}) {
                 g.blocks_per_cu = cap;
                 g.lds_bytes = rtm_hip::occupancy_lds_bytes(b->device, cap);
                 if (g.lds_bytes == 0) continue;
-                const float ms = time_group(half, g);
+                const float ms = time_group(key, g, rounds, reps);
                 if (rc) break;
                 if (ms < 0.98f * best) {
                     best = ms;
@@ -599,11 +637,10 @@ int autotune(rtm_hip_batch* b) {
                 }
             }
         }
-        g.tile_x = pick.tile_x;
-        g.family = pick.family;
-        g.blocks_per_cu = pick.blocks_per_cu;
-        g.lds_bytes = pick.lds_bytes;
+        static_cast<Tuning&>(g) = pick;
+        g.by_load[g.load_class] = pick;
         g.tuned = true;
+        b->stats.tunes += 1;
         b->dirty = true;
     }
     if (t0) (void)hipEventDestroy(t0);
@@ -817,7 +854,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     for (Job* j : b->active) job_free(b, *j);
     for (auto& [id, j] : b->jobs) model_release(b, *j->model);
     for (auto& [id, m] : b->models) model_release(b, *m);
-    for (auto& [half, g] : b->groups)
+    for (auto& [key, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
@@ -928,8 +965,8 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         int warm = 0;
         if (b->cfg.kernel_family == 0)
             for (Job* j : b->active)
-                if (!b->groups[j->model->half].tuned)
-                    warm = std::max(warm, 2 - (2 * j->model->nt - j->steps_left));
+                if (!b->groups[group_key(*j->model)].tuned)
+                    warm = std::max(warm, 2 - (j->desc.steps - j->steps_left));
         if (warm > 0) {
             n = std::min(n, warm);
         } else {
@@ -1028,9 +1065,10 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
     out->bytes_in_use = b->used;
     out->bytes_budget = b->budget;
     int k = 0;
-    for (auto& [half, g] : b->groups) {
-        if (k >= 6) break;
-        out->group_half[k] = half;
+    for (auto& [key, g] : b->groups) {
+        if (k >= 12) break;
+        out->group_half[k] = key_half(key);
+        out->group_snap16[k] = key_snap16(key) ? 1 : 0;
         out->group_tile_x[k] = g.tile_x;
         out->group_family[k] = (int)g.family;
         out->group_waves_per_cu[k] = g.blocks_per_cu * rtm_hip::kWaves;

@@ -24,7 +24,9 @@ struct JobDesc {
     int tile_first;              // its first tile in the group launch
     int bwd_base;                // buffer rotation base of the backward phase
     // clock, advanced on the device after every run of ticks
-    int done;                    // steps done of 2 nt (forward nt, then backward nt)
+    int done;                    // steps done of `steps`
+    int steps;                   // forward nt, then (16-bit snapshots) one
+                                 // bookkeeping tick, then backward nt
     Coeffs coef;
     long long trace_stride;
     long long n_int;
@@ -37,7 +39,13 @@ struct JobDesc {
     float* snap32;               // FP32 snapshots, or
     uint16_t* snap16;            // 16-bit codes +
     float* snap_max;             // per-snapshot max|p|
-    float* wave_max;             // per-tile maxima of the latest stored step
+    // scale bookkeeping of 16-bit snapshots (StepArgs), double-buffered by
+    // snapshot parity: per-tile maxima [2][parts_cap], their slice maxima
+    // [2][kReduceWaves], counts {tiles[2], slices[2]}
+    float* wave_max;
+    float* slice_max;
+    int* part_counts;
+    int parts_cap;
     float* image;
     float* illum;
     const int* tile_ptr[2];      // per-tile injection tables (forward, backward)
@@ -50,18 +58,19 @@ struct JobDesc {
 // kernels.  dry != 0 writes only p^{n+1}: no image, illumination or snapshot
 // update (used to time candidate kernels on live state).  lds_bytes of
 // dynamic LDS per workgroup cap the workgroups resident on a CU.
-hipError_t launch_batch_step(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
+// snap16 selects the build for jobs with 16-bit snapshots (every job of
+// the launch must have the same snapshot width).
+hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, const JobDesc* jobs,
+                             const int* tile_job, int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
+hipError_t launch_batch_step_f32(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
+                                 int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
+hipError_t launch_batch_step_s16(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
+                                 int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.
 hipError_t launch_stack_add(const float* image, const float* illum, int nx, int nz, double* stack_i,
                             double* stack_s, int stack_nz, int ix0, int iz0, hipStream_t stream);
-
-// After tick `sub` of a run: for every job whose step just done was a stored
-// forward step with a 16-bit snapshot, snap_max[slot] = max of its per-tile
-// maxima (one workgroup per job).
-hipError_t launch_batch_reduce(JobDesc* jobs, int njobs, int sub, hipStream_t stream);
 
 // Job clocks: every job has done `ticks` more steps.
 hipError_t launch_batch_advance(JobDesc* jobs, int njobs, int ticks, hipStream_t stream);

@@ -1,0 +1,212 @@
+// batch_launch.hpp -- the batched step kernel (one launch per tick for
+// every job of one stencil half-width; batch_kernels.hip) and its launch
+// dispatch, templated on the snapshot width: batch_kernels_f32.hip and
+// batch_kernels_s16.hip each build one half (jobs with FP32 and with 16-bit
+// snapshots run in separate launches).
+#pragma once
+
+#include "batch_kernels.hpp"
+#include "march_tile.hpp"
+
+namespace rtm_hip {
+namespace {
+
+using namespace device;
+
+__device__ __forceinline__ float* rotate(const JobDesc& J, int r) {
+    return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
+}
+
+// Step arguments of job J after `total` of its steps (forward steps
+// 0..nt-1, the bookkeeping tick of a 16-bit job, then backward steps
+// 0..nt-1).  The stencil coefficients are read from the descriptor where they
+// are used (an array copied into a local struct would be kept in scratch
+// memory).
+template <bool S16>
+__device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dry) {
+    StepArgs a{};
+    constexpr bool wide = !S16;
+    const int fwd = total < J.nt;
+    const int s = fwd ? total : total - J.nt - (wide ? 0 : 1);
+    const int base = fwd ? 0 : J.bwd_base;
+    a.p_prev = rotate(J, (base + s) % 3);
+    a.p_cur = rotate(J, (base + s + 1) % 3);
+    a.p_next = rotate(J, (base + s + 2) % 3);
+    a.zero_prev = s <= 1;
+    a.zero_cur = s == 0;
+    a.vdt2 = J.vdt2;
+    a.wx = J.wx;
+    a.wz = J.wz;
+    a.nxe = J.nxe; a.nze = J.nze; a.ldz = J.ldz; a.nb = J.nb; a.nx = J.nx; a.nz = J.nz;
+    a.ntz = J.ntz; a.ntx = J.ntx; a.ntiles = J.ntiles;
+    const int it = fwd ? s : J.nt - 1 - s;
+    a.it = it;
+    a.tile_ptr = fwd ? J.tile_ptr[0] : J.tile_ptr[1];
+    a.entries = fwd ? J.entries[0] : J.entries[1];
+    a.wavelet = J.wavelet;
+    a.traces = J.traces;
+    a.trace_stride = J.trace_stride;
+    a.image = J.image;
+    a.illum = J.illum;
+    a.dry = dry;
+    a.snap_fmt = J.snap_fmt;
+    // Every field is assigned once, unconditionally: stores of different
+    // fields in different branches can be merged into one store at a
+    // computed offset, which would keep the whole struct in scratch memory.
+    const int st = J.store;
+    const bool stored = (it % st) == 0;
+    const long long slot = it / st;
+    const bool half16 = !wide && fwd;
+    const bool red = half16 && it >= 1 && ((it - 1) % st) == 0;          // slices of step it-1's maxima
+    const bool quant = half16 && it >= 2 && ((it - 2) % st) == 0;        // codes of step it-2
+    const int p0 = (int)(slot & 1), p1 = (int)(((it - 1) / st) & 1), p2 = (int)(((it - 2) / st) & 1);
+    // the last forward field, imaged at backward step 0 (its slices were
+    // reduced by the bookkeeping tick)
+    const bool pending = !wide && !fwd && stored && s == 0;
+    a.snap_out = (wide && fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
+    a.snap_in = (wide && !fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
+    a.wave_max = (half16 && stored) ? J.wave_max + (ptrdiff_t)p0 * J.parts_cap : nullptr;
+    a.wave_count = (half16 && stored) ? J.part_counts + p0 : nullptr;
+    a.red_in = red ? J.wave_max + (ptrdiff_t)p1 * J.parts_cap : nullptr;
+    a.red_count = red ? J.part_counts + p1 : nullptr;
+    a.red_out = red ? J.slice_max + p1 * kReduceWaves : nullptr;
+    a.red_out_count = red ? J.part_counts + 2 + p1 : nullptr;
+    a.quant_parts = quant ? J.slice_max + p2 * kReduceWaves : (pending ? J.slice_max + p0 * kReduceWaves : nullptr);
+    a.quant_count = quant ? J.part_counts + 2 + p2 : (pending ? J.part_counts + 2 + p0 : nullptr);
+    a.quant_out = quant ? J.snap16 + (it - 2) / st * J.n_int : nullptr;
+    a.quant_max_out = quant ? J.snap_max + (it - 2) / st : nullptr;
+    a.parts_cap = J.parts_cap;
+    a.snap16_max = (!wide && !fwd && stored && !pending) ? J.snap_max + slot : nullptr;
+    a.pending_field = pending ? rotate(J, J.bwd_base) : nullptr;
+    a.snap16_in = (!wide && !fwd && stored && !pending) ? J.snap16 + slot * J.n_int : nullptr;
+    return a;
+}
+
+// The bookkeeping tick of a 16-bit job, between its phases (wave t of the
+// job's TX-column tiles): the stored forward steps among the last two have
+// no step it+2 to settle their scale.  Step nt-2: its codes, from its field
+// (still in its buffer), with the max of its slices; step nt-1: the slices of
+// its per-tile maxima, which backward step 0 reduces to the max it images
+// with.
+template <int H, int TX>
+__device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int lane) {
+    constexpr int TZ = kTileZ - 2 * H;
+    const int st = J.store, nt = J.nt;
+    if (nt >= 2 && ((nt - 2) % st) == 0) {
+        const int slot = (nt - 2) / st, p = slot & 1;
+        const float m = parts_max(J.slice_max + p * kReduceWaves, J.part_counts + 2 + p, lane);
+        const int k = snapshot_scale_exponent(m);
+        if (t == 0 && lane == 0) gbl(J.snap_max)[slot] = m;
+        const float* f = rotate(J, nt % 3);          // buffer of forward step nt-2's field
+        uint16_t* out = J.snap16 + (ptrdiff_t)slot * J.n_int;
+        const int tz = t % J.ntz, tx = t / J.ntz;
+        const int z = tz * TZ + lane;
+        const int jz = z - J.nb;
+        const bool z_int = lane >= H && lane < kTileZ - H && z < J.nze - H && jz >= 0 && jz < J.nz;
+#pragma unroll
+        for (int r = 0; r < TX; ++r) {
+            const int x = H + tx * TX + r, jx = x - J.nb;
+            if (z_int && x < J.nxe - H && jx >= 0 && jx < J.nx)
+                gbl(out)[(ptrdiff_t)jx * J.nz + jz] =
+                    encode_snapshot(gbl(f)[(ptrdiff_t)x * J.ldz + z], k, J.snap_fmt);
+        }
+    }
+    if (((nt - 1) % st) == 0) {
+        const int p = ((nt - 1) / st) & 1;
+        reduce_slice(J.wave_max + (ptrdiff_t)p * J.parts_cap, J.part_counts + p, J.parts_cap,
+                     J.slice_max + p * kReduceWaves, J.part_counts + 2 + p, t, J.ntiles, lane);
+    }
+}
+
+// Tick `sub` of a run of ticks for every job of half-width H: job J is at
+// step J.done + sub.  The job and tile of a wave are wave-uniform
+// (read-first-lane), so the descriptor is read with scalar loads and the
+// step arguments are built in scalar registers, as a kernel argument would
+// be.
+template <int H, int TX, Family F, bool S16>
+__global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
+                                                         const int* __restrict__ tile_job, int ntiles, int sub,
+                                                         int dry) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const int nblocks = (ntiles + kWaves - 1) / kWaves;
+    const int g = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
+    if (g >= ntiles) return;
+    const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
+    const JobDesc& J = jobs[jid];
+    const int total = J.done + sub;
+    if (total >= J.steps) return;            // finished job: its tiles idle
+    if (S16 && total == J.nt) {
+        if (!dry) bookkeeping_tile<H, TX>(J, g - J.tile_first, lane);
+        return;
+    }
+    const StepArgs a = job_args<S16>(J, total, dry);
+    const int t = g - J.tile_first;
+    if constexpr (F == Family::kStream) {
+        if (total < J.nt) stream_tile<H, TX, Dir::Forward, S16>(a, J.coef, t, lane);
+        else stream_tile<H, TX, Dir::Backward, S16>(a, J.coef, t, lane);
+    } else {
+        constexpr bool DPP = F == Family::kMarchDpp;
+        if (total < J.nt) march_tile<H, TX, Dir::Forward, DPP, S16>(a, J.coef, t, lane);
+        else march_tile<H, TX, Dir::Backward, DPP, S16>(a, J.coef, t, lane);
+    }
+}
+
+template <int H, bool S16>
+hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* tile_job, int ntiles, int sub,
+                    int dry, size_t lds, hipStream_t s) {
+    const dim3 block(kBlockThreads);
+    const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
+#define RTM_BATCH(TX, F) \
+    hipLaunchKernelGGL((k_batch<H, TX, F, S16>), grid, block, lds, s, jobs, tile_job, ntiles, sub, dry)
+    if (family == Family::kStream) {
+        switch (tile_x) {
+            case 8: RTM_BATCH(8, Family::kStream); break;
+            case 16: RTM_BATCH(16, Family::kStream); break;
+            case 32: RTM_BATCH(32, Family::kStream); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (family == Family::kMarchDpp) {
+        switch (tile_x) {
+            case 2: RTM_BATCH(2, Family::kMarchDpp); break;
+            case 3: RTM_BATCH(3, Family::kMarchDpp); break;
+            case 4: RTM_BATCH(4, Family::kMarchDpp); break;
+            case 5: RTM_BATCH(5, Family::kMarchDpp); break;
+            case 6: RTM_BATCH(6, Family::kMarchDpp); break;
+            case 8: RTM_BATCH(8, Family::kMarchDpp); break;
+            case 16: RTM_BATCH(16, Family::kMarchDpp); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (family == Family::kMarch) {
+        switch (tile_x) {
+            case 4: RTM_BATCH(4, Family::kMarch); break;
+            case 8: RTM_BATCH(8, Family::kMarch); break;
+            case 16: RTM_BATCH(16, Family::kMarch); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef RTM_BATCH
+    return hipGetLastError();
+}
+
+
+template <bool S16>
+hipError_t launch_batch_step_t(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
+                             int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream) {
+    const size_t lds = (size_t)lds_bytes;
+    if (ntiles <= 0) return hipSuccess;
+    switch (half) {
+        case 1: return launch_h<1, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 2: return launch_h<2, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 3: return launch_h<3, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 4: return launch_h<4, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 5: return launch_h<5, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 6: return launch_h<6, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+}  // namespace rtm_hip

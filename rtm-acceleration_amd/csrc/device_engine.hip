@@ -68,8 +68,13 @@ struct rtm_hip_engine {
     float* d_snap = nullptr;         // FP32 snapshots
     uint16_t* d_snap16 = nullptr;    // FP16/BF16 snapshot codes
     float* d_snap_max = nullptr;     // per-snapshot max|p| (16-bit formats)
-    float* d_wave_max = nullptr;     // per-wave maxima of the latest stored step
+    // scale bookkeeping of 16-bit snapshots (StepArgs), double-buffered by
+    // snapshot parity: per-wave maxima [2][wave_max_cap], their slice maxima
+    // [2][kReduceWaves], and the counts {waves[2], slices[2]}
+    float* d_wave_max = nullptr;
     int wave_max_cap = 0;
+    float* d_slice_max = nullptr;
+    int* d_part_counts = nullptr;
     size_t snap_bytes = 0;
     float* d_image = nullptr;
     float* d_illum = nullptr;
@@ -157,7 +162,7 @@ struct rtm_hip_engine {
         };
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
-        f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_slice_max); f(d_part_counts); f(d_image); f(d_illum); f(d_traces); f(d_rec);
         f(d_stack_i); f(d_stack_s);
         for (int k = 0; k < 2; ++k) {
             f(d_tile_ptr[k]); f(d_ent[k]);
@@ -323,27 +328,50 @@ void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepAr
     a.snap_fmt = h->snap_fmt;
     a.snap_out = nullptr;
     a.wave_max = nullptr;
+    a.wave_count = nullptr;
+    a.red_in = nullptr;
+    a.red_count = nullptr;
+    a.red_out = nullptr;
+    a.red_out_count = nullptr;
+    a.quant_parts = nullptr;
+    a.quant_count = nullptr;
     a.quant_out = nullptr;
-    a.quant_max = nullptr;
+    a.quant_max_out = nullptr;
+    a.parts_cap = h->wave_max_cap;
     a.snap_in = nullptr;
     a.snap16_in = nullptr;
     a.snap16_max = nullptr;
-    const bool stored = (it % h->store) == 0;
-    const size_t slot = (size_t)(it / h->store);
+    a.pending_field = nullptr;
+    const int st = h->store;
+    const bool stored = (it % st) == 0;
+    const size_t slot = (size_t)(it / st);
     if (h->snap_fmt == RTM_HIP_SNAP_F32) {
         if (!stored) return;
         if (dir == Dir::Forward) a.snap_out = h->d_snap + slot * h->n_int;
         else a.snap_in = h->d_snap + slot * h->n_int;
         return;
     }
+    constexpr int kRW = rtm_hip::kReduceWaves;
     if (dir == Dir::Forward) {
-        // per-wave maxima of the new field; run_loop reduces them after the launch
-        if (stored) a.wave_max = h->d_wave_max;
-        // p^n of this launch is the field of step it-1: quantise it if stored
-        if (it >= 1 && ((it - 1) % h->store) == 0) {
-            const size_t prev = (size_t)((it - 1) / h->store);
-            a.quant_out = h->d_snap16 + prev * h->n_int;
-            a.quant_max = h->d_snap_max + prev;
+        if (stored) {                                   // per-wave maxima of the new field
+            const int p = (int)(slot & 1);
+            a.wave_max = h->d_wave_max + (size_t)p * h->wave_max_cap;
+            a.wave_count = h->d_part_counts + p;
+        }
+        if (it >= 1 && ((it - 1) % st) == 0) {          // their slices, one step later
+            const int p = (int)(((it - 1) / st) & 1);
+            a.red_in = h->d_wave_max + (size_t)p * h->wave_max_cap;
+            a.red_count = h->d_part_counts + p;
+            a.red_out = h->d_slice_max + p * kRW;
+            a.red_out_count = h->d_part_counts + 2 + p;
+        }
+        if (it >= 2 && ((it - 2) % st) == 0) {          // the codes, two steps later
+            const size_t s2 = (size_t)((it - 2) / st);
+            const int p = (int)(s2 & 1);
+            a.quant_parts = h->d_slice_max + p * kRW;
+            a.quant_count = h->d_part_counts + 2 + p;
+            a.quant_out = h->d_snap16 + s2 * h->n_int;
+            a.quant_max_out = h->d_snap_max + s2;
         }
     } else if (stored) {
         a.snap16_in = h->d_snap16 + slot * h->n_int;
@@ -410,19 +438,6 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
         if (sample) h->ev_pending.push_back({k, ev, 1});
         h->launches[k] += 1;
-        // A stored 16-bit step: reduce the per-wave maxima to the snapshot's
-        // max; the next step, which reads the new field anyway as p^n, writes
-        // its codes (the last step's field is quantised by its own pass).
-        if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32 && (it % h->store) == 0) {
-            const size_t slot = (size_t)(it / h->store);
-            const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
-            e = rtm_hip::launch_reduce_max(h->d_wave_max, nparts, h->d_snap_max + slot, h->stream);
-            if (e == hipSuccess && it == h->nt - 1)
-                e = rtm_hip::launch_quantise(h->d_fld[ic], h->ldz, h->nb, h->nx, h->nz,
-                                             static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
-                                             h->d_snap16 + slot * h->n_int, h->stream);
-            if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
-        }
         // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
         const int old = ia;
         ia = ib;
@@ -432,6 +447,29 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     if (phase_ev.start) {
         (void)hipEventRecord(phase_ev.stop, h->stream);
         h->ev_pending.push_back({k, phase_ev, h->nt});
+    }
+    // 16-bit stored steps among the last two have no step it+2 in the loop:
+    // their max from the slices (step nt-2) or the per-wave maxima (step
+    // nt-1), then their codes from the field (p_cur / p_prev after the loop's
+    // last rotation hold the fields of steps nt-1 / nt-2)
+    if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32) {
+        const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
+        for (int it = std::max(0, h->nt - 2); it < h->nt; ++it) {
+            if (it % h->store) continue;
+            const size_t slot = (size_t)(it / h->store);
+            const int p = (int)(slot & 1);
+            const bool last = it == h->nt - 1;
+            hipError_t e = last ? rtm_hip::launch_reduce_max(h->d_wave_max + (size_t)p * h->wave_max_cap, nparts,
+                                                             h->d_snap_max + slot, h->stream)
+                                : rtm_hip::launch_reduce_max(h->d_slice_max + p * rtm_hip::kReduceWaves,
+                                                             std::min(rtm_hip::kReduceWaves, nparts),
+                                                             h->d_snap_max + slot, h->stream);
+            if (e == hipSuccess)
+                e = rtm_hip::launch_quantise(h->d_fld[last ? ib : ia], h->ldz, h->nb, h->nx, h->nz,
+                                             static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
+                                             h->d_snap16 + slot * h->n_int, h->stream);
+            if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
+        }
     }
     if (dir == Dir::Backward && h->image_stack) {
         hipError_t e = rtm_hip::launch_fold_fp64(h->d_image, h->d_illum, h->d_stack_i, h->d_stack_s, h->n_int,
@@ -762,7 +800,8 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (h->snap_fmt == RTM_HIP_SNAP_F32) {
         if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
     } else {
-        if (!rc) rc = h->alloc(&h->d_snap16, h->nsnap * h->n_int, "snapshot store");
+        // one spare word: imaging reads the aligned 32-bit word holding a code
+        if (!rc) rc = h->alloc(&h->d_snap16, h->nsnap * h->n_int + 2, "snapshot store");
         if (!rc) rc = h->alloc(&h->d_snap_max, (size_t)h->nsnap, "snapshot scales");
         // per-wave maxima: as many as the largest launch of any step-kernel mapping
         int parts = 0;
@@ -772,7 +811,11 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
                 const rtm_hip::StepGeometry g = rtm_hip::step_geometry(f, h->half, tx, h->nxe, h->nze, h->ldz);
                 parts = std::max(parts, f == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles);
             }
-        if (!rc) rc = h->alloc(&h->d_wave_max, (size_t)parts, "snapshot wave maxima");
+        if (!rc) rc = h->alloc(&h->d_wave_max, 2 * (size_t)parts, "snapshot wave maxima");
+        if (!rc) rc = h->alloc(&h->d_slice_max, 2 * (size_t)rtm_hip::kReduceWaves, "snapshot slice maxima");
+        if (!rc) rc = h->alloc(&h->d_part_counts, 4, "snapshot maxima counts");
+        if (!rc && hipMemset(h->d_part_counts, 0, 4 * sizeof(int)) != hipSuccess)
+            rc = h->fail(RTM_HIP_ERR_DEVICE, "snapshot maxima counts: clearing failed");
         h->wave_max_cap = parts;
     }
     if (h->image_stack) {

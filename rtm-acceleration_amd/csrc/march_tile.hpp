@@ -21,6 +21,18 @@ __device__ __forceinline__ global_t<T>* gbl(T* p) {
     return (global_t<T>*)p;
 }
 
+// Values that no wave of the launch writes (tile tables, and the scales and
+// counts of earlier launches) are read through the constant address space,
+// so a wave-uniform read becomes a scalar load: it waits on its own counter,
+// and using the value does not wait for the vector loads still in flight
+// (the vector counter retires in order).
+template <typename T>
+using const_t = __attribute__((address_space(4))) const T;
+template <typename T>
+__device__ __forceinline__ T sld(const T* p) {
+    return *(const_t<T>*)p;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Give XCD k the
 // k-th contiguous block of logical tiles so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
@@ -43,6 +55,31 @@ __device__ __forceinline__ float wave_max(float m) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, kTileZ));
     return m;
+}
+
+// max(0, parts[0..n)) over a workgroup of kThreads threads, in thread 0.  Each
+// thread issues its loads four at a time (independent loads in flight, one
+// round trip per four values instead of one per value); max is exact, so the
+// order of the reduction does not change the result.
+template <int kThreads>
+__device__ __forceinline__ float block_max(const float* parts, int n) {
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int i = (int)threadIdx.x;
+    for (; i + 3 * kThreads < n; i += 4 * kThreads) {
+        const float a = gbl(parts)[i], b = gbl(parts)[i + kThreads];
+        const float c = gbl(parts)[i + 2 * kThreads], d = gbl(parts)[i + 3 * kThreads];
+        m0 = fmaxf(m0, a);
+        m1 = fmaxf(m1, b);
+        m2 = fmaxf(m2, c);
+        m3 = fmaxf(m3, d);
+    }
+    for (; i < n; i += kThreads) m0 = fmaxf(m0, gbl(parts)[i]);
+    float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
+    __shared__ float wm[kThreads / kTileZ];
+    if ((threadIdx.x & (kTileZ - 1)) == 0) wm[threadIdx.x / kTileZ] = m;
+    __syncthreads();
+    m = threadIdx.x < kThreads / kTileZ ? wm[threadIdx.x] : 0.0f;
+    return wave_max(m);
 }
 
 // v * 2^k rounded to FP16 or BF16 (round to nearest even).
@@ -122,37 +159,101 @@ __device__ __forceinline__ float apply_entries(const StepArgs& a, const TileEntr
     return out;
 }
 
-// What the snapshot/imaging epilogue does on this launch.
+// What the snapshot/imaging epilogue does on this launch.  Every kernel is
+// built twice: for FP32 snapshots (S16 false) and for 16-bit ones (S16
+// true), so neither carries the other's code.
 struct SnapCtx {
     bool imaging;      // backward stored step
     bool store32;      // forward stored step, FP32 snapshot
-    bool store_max;    // forward stored step, 16-bit snapshot (max reduction)
-    bool quantise;     // forward, previous step was stored (16-bit)
+    bool store_max;    // forward stored step, 16-bit snapshot (per-wave max)
+    bool quantise;     // forward, two steps after a stored step (16-bit)
     int k_in;          // backward decode exponent
     int k_q;           // forward encode exponent
+    float q_max;       // the quantised snapshot's max
 };
 
+// max of parts[0 .. *count) (at most kReduceWaves values), in every lane.
+__device__ __forceinline__ float parts_max(const float* parts, const int* count, int lane) {
+    const int n = min(sld(count), kReduceWaves);
+    return wave_max(lane < n ? gbl(parts)[lane] : 0.0f);
+}
+
+// The epilogue's flags (no memory access).
+template <bool S16>
 __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
     SnapCtx c;
-    const bool wide = a.snap_fmt == (int)SnapFmt::F32;
-    c.imaging = d == Dir::Backward &&
-                (wide ? a.snap_in != nullptr : (a.snap16_in != nullptr || a.pending_field != nullptr));
-    c.store32 = d == Dir::Forward && wide && a.snap_out != nullptr;
-    c.store_max = d == Dir::Forward && !wide && a.wave_max != nullptr;
-    c.quantise = d == Dir::Forward && !wide && a.quant_out != nullptr;
-    c.k_in = (c.imaging && !wide) ? snapshot_scale_exponent(*gbl(a.snap16_max)) : 0;
-    c.k_q = c.quantise ? snapshot_scale_exponent(*gbl(a.quant_max)) : 0;
+    if constexpr (S16) {
+        c.imaging = d == Dir::Backward && (a.snap16_in != nullptr || a.pending_field != nullptr);
+        c.store32 = false;
+        c.store_max = d == Dir::Forward && a.wave_max != nullptr;
+        c.quantise = d == Dir::Forward && a.quant_out != nullptr;
+    } else {
+        c.imaging = d == Dir::Backward && a.snap_in != nullptr;
+        c.store32 = d == Dir::Forward && a.snap_out != nullptr;
+        c.store_max = c.quantise = false;
+    }
+    c.k_in = c.k_q = 0;
+    c.q_max = 0.0f;
     return c;
 }
 
-// Snapshot value f for imaging at interior index j / extended index i: FP32,
-// a decoded 16-bit code, or a 16-bit snapshot still pending in its field
-// (encoded and decoded here, which gives the value the store would hold).
-__device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx& c, ptrdiff_t j, ptrdiff_t i) {
-    if (a.snap_fmt == (int)SnapFmt::F32) return gbl(a.snap_in)[j];
-    const uint16_t code =
-        a.pending_field ? encode_snapshot(gbl(a.pending_field)[i], c.k_in, a.snap_fmt) : gbl(a.snap16_in)[j];
-    return decode_snapshot(code, c.k_in, (SnapFmt)a.snap_fmt);
+// The 16-bit scales of the epilogue.  Called after the tile's loads are
+// issued: the reduction waits for its own load, and so for all before it.
+template <bool S16>
+__device__ __forceinline__ void snap_scales(const StepArgs& a, SnapCtx& c, int lane) {
+    if constexpr (S16) {
+        if (c.imaging)
+            c.k_in = snapshot_scale_exponent(a.pending_field ? parts_max(a.quant_parts, a.quant_count, lane)
+                                                             : sld(a.snap16_max));
+        if (c.quantise) {
+            c.q_max = parts_max(a.quant_parts, a.quant_count, lane);
+            c.k_q = snapshot_scale_exponent(c.q_max);
+        }
+    }
+}
+
+// The stored bits of the snapshot value for imaging at interior index j /
+// extended index i: an FP32 value, or for 16-bit snapshots the aligned
+// 32-bit word holding the code (the 16-bit stores are allocated with one
+// spare word at the end) or, for a snapshot still pending in its field, the
+// field's FP32 value.  One 32-bit load from a uniformly selected address (two
+// loads in uniform branches would need a wait between them, as they share a
+// destination), and no use of the value here: the decode is left to
+// snapshot_value, after every load of the tile is issued (a use here would
+// wait for this load, and with it for every load before it).
+template <bool S16>
+__device__ __forceinline__ uint32_t snapshot_bits(const StepArgs& a, ptrdiff_t j, ptrdiff_t i) {
+    if constexpr (!S16) return __float_as_uint(gbl(a.snap_in)[j]);
+    const uintptr_t addr = a.pending_field ? (uintptr_t)(a.pending_field + i) : (uintptr_t)(a.snap16_in + j);
+    return *gbl((const uint32_t*)(addr & ~(uintptr_t)3));
+}
+
+// 16-bit code -> the stored float (exact): the hardware conversion for FP16,
+// a shift for BF16, then the exact scale 2^-k.
+__device__ __forceinline__ float decode_code(uint32_t code, int k, int fmt) {
+    float f;
+    if (fmt == (int)SnapFmt::BF16) {
+        f = __uint_as_float(code << 16);
+    } else {
+        const uint16_t c16 = (uint16_t)code;
+        _Float16 h;
+        __builtin_memcpy(&h, &c16, sizeof h);
+        f = (float)h;
+    }
+    return ldexpf(f, -k);
+}
+
+// Snapshot value f from its stored bits (snapshot_bits, interior index j):
+// FP32 as is, a decoded 16-bit code (the half of the word that holds element
+// j), or a pending field value encoded and decoded here, which gives the
+// value the store would hold.
+template <bool S16>
+__device__ __forceinline__ float snapshot_value(const StepArgs& a, const SnapCtx& c, uint32_t bits, ptrdiff_t j) {
+    if constexpr (!S16) return __uint_as_float(bits);
+    uint32_t code;
+    if (a.pending_field) code = encode_snapshot(__uint_as_float(bits), c.k_in, a.snap_fmt);
+    else code = ((uintptr_t)(a.snap16_in + j) & 2) ? bits >> 16 : bits & 0xffffu;
+    return decode_code(code, c.k_in, a.snap_fmt);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,10 +290,11 @@ __device__ __forceinline__ float laplacian(const Coeffs& cf, const float* q, int
 // taper and, for a backward imaging step, the snapshot value and the image
 // and illumination being accumulated.
 struct ColumnOps {
-    float pv, vv, wx, fv, iv, sv;
+    float pv, vv, wx, iv, sv;
+    uint32_t fb;                     // snapshot bits (snapshot_bits)
 };
 
-template <int H, Dir D>
+template <int H, Dir D, bool S16>
 __device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_out,
                                                 bool z_int, int jz) {
     ColumnOps o;
@@ -201,12 +303,13 @@ __device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx
     o.pv = (ok && !a.zero_prev) ? gbl(a.p_prev)[i] : 0.0f;
     o.vv = ok ? gbl(a.vdt2)[i] : 0.0f;
     o.wx = x < a.nxe ? gbl(a.wx)[x] : 0.0f;
-    o.fv = o.iv = o.sv = 0.0f;
+    o.iv = o.sv = 0.0f;
+    o.fb = 0;
     if constexpr (D == Dir::Backward) {
         const int jx = x - a.nb;
         const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
         const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-        o.fv = in ? snapshot_value(a, sc, j, i) : 0.0f;
+        o.fb = in ? snapshot_bits<S16>(a, j, i) : 0u;
         o.iv = in ? gbl(a.image)[j] : 0.0f;
         o.sv = in ? gbl(a.illum)[j] : 0.0f;
     }
@@ -225,7 +328,7 @@ __device__ __forceinline__ float load_cur(const StepArgs& a, int x, int z, bool 
 // previous step, which gives the same value; injection cells lie where
 // w == 1, so injecting after the damping multiply is the reference's
 // inject-then-damp).  Entries [e0, e1) may hit this cell.
-template <Dir D>
+template <Dir D, bool S16>
 __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
                                             float pc, float lap, const ColumnOps& o, float wzv,
                                             const TileEntries& te, int e0, int e1, float& vmax) {
@@ -241,31 +344,81 @@ __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc
     if (z_int && jx >= 0 && jx < a.nx) {
         const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
         if constexpr (D == Dir::Forward) {
-            if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
-            if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-            if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(pc, sc.k_q, a.snap_fmt);
+            if constexpr (S16) {
+                if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
+                if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(o.pv, sc.k_q, a.snap_fmt);
+            } else {
+                if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
+            }
         } else {
             if (sc.imaging && !a.dry) {
-                gbl(a.image)[j] = o.iv + o.fv * out;
-                gbl(a.illum)[j] = o.sv + o.fv * o.fv;
+                const float fv = snapshot_value<S16>(a, sc, o.fb, j);
+                gbl(a.image)[j] = o.iv + fv * out;
+                gbl(a.illum)[j] = o.sv + fv * fv;
             }
         }
     }
 }
 
-// The wave's max |p^{n+1}| of a stored 16-bit step, to partial `part`.
-__device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc, float vmax, int lane, int part) {
+// The wave's max |p^{n+1}| of a stored 16-bit step, to partial `part` of
+// the launch's `nparts`.
+template <bool S16>
+__device__ __forceinline__ void finish_tile(const StepArgs& a, const SnapCtx& sc, float vmax, int lane, int part,
+                                            int nparts) {
+    if (!S16) return;
     if (sc.store_max && !a.dry) {
         vmax = wave_max(vmax);
-        if (lane == 0) gbl(a.wave_max)[part] = vmax;
+        if (lane == 0) {
+            gbl(a.wave_max)[part] = vmax;
+            if (part == 0) *gbl(a.wave_count) = nparts;
+        }
     }
+}
+
+// Wave w of a launch of nwaves: out[w] = max of slice w of in[0 .. *count)
+// (count clamped to cap), for w < r = min(kReduceWaves, nwaves), in r slices;
+// wave 0 writes r to *out_count.
+__device__ __forceinline__ void reduce_slice(const float* in, const int* count, int cap, float* out, int* out_count,
+                                             int w, int nwaves, int lane) {
+    const int r = min(kReduceWaves, nwaves);
+    if (w >= r) return;
+    const int n = min(sld(count), cap);
+    const int chunk = (n + r - 1) / r;
+    const int lo = w * chunk, hi = min(n, lo + chunk);
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int i = lo + lane;
+    for (; i + 3 * kTileZ < hi; i += 4 * kTileZ) {
+        const float x0 = gbl(in)[i], x1 = gbl(in)[i + kTileZ];
+        const float x2 = gbl(in)[i + 2 * kTileZ], x3 = gbl(in)[i + 3 * kTileZ];
+        m0 = fmaxf(m0, x0);
+        m1 = fmaxf(m1, x1);
+        m2 = fmaxf(m2, x2);
+        m3 = fmaxf(m3, x3);
+    }
+    for (; i < hi; i += kTileZ) m0 = fmaxf(m0, gbl(in)[i]);
+    const float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
+    if (lane == 0) {
+        gbl(out)[w] = m;
+        if (w == 0) *gbl(out_count) = r;
+    }
+}
+
+// The scale bookkeeping of the launch's wave `w` of `nwaves`: one slice of
+// the previous stored step's per-wave maxima, and the quantised snapshot's
+// max (wave 0).
+template <bool S16>
+__device__ __forceinline__ void settle_scales(const StepArgs& a, const SnapCtx& sc, int w, int nwaves, int lane) {
+    if (!S16 || a.dry) return;
+    if (sc.quantise && w == 0 && lane == 0) *gbl(a.quant_max_out) = sc.q_max;
+    if (a.red_in != nullptr)
+        reduce_slice(a.red_in, a.red_count, a.parts_cap, a.red_out, a.red_out_count, w, nwaves, lane);
 }
 
 // ---------------------------------------------------------------------------
 // March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
-template <int H, int TX, Dir D, bool DPP>
+template <int H, int TX, Dir D, bool DPP, bool S16>
 __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
@@ -277,17 +430,18 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
     const int jz = z - a.nb;                 // interior depth index
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
-    const SnapCtx sc = snap_ctx(a, D);
+    SnapCtx sc = snap_ctx<S16>(a, D);
 
     float q[NQ];
 #pragma unroll
     for (int j = 0; j < NQ; ++j) q[j] = load_cur(a, x0 - H + j, z, z_load);
     ColumnOps ops[TX];
 #pragma unroll
-    for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D>(a, sc, x0 + r, z, z_out, z_int, jz);
+    for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D, S16>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
-    const int e0 = gbl(a.tile_ptr)[t], e1 = gbl(a.tile_ptr)[t + 1];
+    const int e0 = sld(a.tile_ptr + t), e1 = sld(a.tile_ptr + t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
+    snap_scales<S16>(a, sc, lane);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -295,9 +449,10 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         const float lap = laplacian<H, DPP>(cf, q, r + H);
         const int x = x0 + r;
         if (!z_out || x >= a.nxe - H) continue;
-        finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, e0, e1, vmax);
+        finish_cell<D, S16>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, e0, e1, vmax);
     }
-    finish_tile(a, sc, vmax, lane, t);
+    finish_tile<S16>(a, sc, vmax, lane, t, a.ntiles);
+    settle_scales<S16>(a, sc, t, a.ntiles, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +464,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
 // sorted by cell (column-major), so a uniform cursor hands each column its
 // own entries.
 // ---------------------------------------------------------------------------
-template <int H, int XS, Dir D>
+template <int H, int XS, Dir D, bool S16>
 __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int C = 4;                     // columns per chunk
     constexpr int NCH = XS / C;
@@ -324,18 +479,19 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
     const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
     const int jz = z - a.nb;
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
-    const SnapCtx sc = snap_ctx(a, D);
+    SnapCtx sc = snap_ctx<S16>(a, D);
 
     float q[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) q[j] = load_cur(a, x0 - H + j, z, z_load);
     ColumnOps ops[C];
 #pragma unroll
-    for (int r = 0; r < C; ++r) ops[r] = column_ops<H, D>(a, sc, x0 + r, z, z_out, z_int, jz);
+    for (int r = 0; r < C; ++r) ops[r] = column_ops<H, D, S16>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
-    int ec = gbl(a.tile_ptr)[t];
-    const int e1 = gbl(a.tile_ptr)[t + 1];
+    int ec = sld(a.tile_ptr + t);
+    const int e1 = sld(a.tile_ptr + t + 1);
     const TileEntries te = load_entries<D>(a, ec, e1, lane);
+    snap_scales<S16>(a, sc, lane);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -348,7 +504,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
 #pragma unroll
             for (int j = 0; j < C; ++j) nq[j] = load_cur(a, xc + C + H + j, z, z_load);
 #pragma unroll
-            for (int r = 0; r < C; ++r) nops[r] = column_ops<H, D>(a, sc, xc + C + r, z, z_out, z_int, jz);
+            for (int r = 0; r < C; ++r) nops[r] = column_ops<H, D, S16>(a, sc, xc + C + r, z, z_out, z_int, jz);
         }
 #pragma unroll
         for (int r = 0; r < C; ++r) {
@@ -359,7 +515,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
             while (ee < e1 && entry_cell(a, te, ee) < col_end) ++ee;
             const float lap = laplacian<H, true>(cf, q, r + H);
             if (z_out && x < a.nxe - H)
-                finish_cell<D>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, ec, ee, vmax);
+                finish_cell<D, S16>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, ec, ee, vmax);
             ec = ee;
         }
         if (k + 1 < NCH) {
@@ -371,7 +527,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
             for (int r = 0; r < C; ++r) ops[r] = nops[r];
         }
     }
-    finish_tile(a, sc, vmax, lane, t);
+    finish_tile<S16>(a, sc, vmax, lane, t, a.ntiles);
+    settle_scales<S16>(a, sc, t, a.ntiles, lane);
 }
 
 }  // namespace device

@@ -15,6 +15,7 @@ constexpr int kMaxHalf = 6;
 constexpr int kTileZ = 64;        // one wave64 spans 64 consecutive depth cells
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kBlockThreads = kTileZ * kWaves;
+constexpr int kReduceWaves = 64;  // waves of a launch that share a snapshot-max reduction
 
 
 // Stencil coefficients pre-divided by the spacing (src/rtm/rtm_cpu.cpp:75-79).
@@ -61,16 +62,33 @@ struct StepArgs {
     float*       rec_out;     // forward modelling: trace-major, stride rec_stride
     long long    rec_stride;
     int          snap_fmt;    // SnapFmt
-    // forward, stored step: FP32 -> the new field's interior into snap_out;
-    // F16/BF16 -> each wave's max|p^{n+1}| over its interior cells into
-    // wave_max[w] (w = the wave's tile, or tile*kWaves + wave for the LDS
-    // mapping); reduce_max_parts then gives the snapshot's max
+    // forward, stored step, FP32: the new field's interior into snap_out
     float*       snap_out;
+    // F16/BF16 snapshots.  The scale of the snapshot of stored forward step
+    // `it` (max|p| over the interior) is settled inside the next two step
+    // launches, with no launch of its own:
+    //   step it:   wave w of the launch writes max|p^{n+1}| over its interior
+    //              cells to wave_max[w]; wave 0 writes the launch's wave
+    //              count to *wave_count
+    //   step it+1: waves w < min(kReduceWaves, waves of the launch) each
+    //              reduce a slice of red_in[0 .. *red_count) (clamped to
+    //              parts_cap) to red_out[w]; wave 0 writes the slice count
+    //              to *red_out_count
+    //   step it+2: every wave reduces quant_parts[0 .. *quant_count) to the
+    //              snapshot's max, quantises p^{n-1} (the field of step it,
+    //              which it loads anyway) into quant_out, and wave 0 writes
+    //              the max to *quant_max_out for the backward decode
     float*       wave_max;
-    // forward, step after a stored step (F16/BF16): quantise p^n's interior
-    // into quant_out with the scale of *quant_max
+    int*         wave_count;
+    const float* red_in;
+    const int*   red_count;
+    float*       red_out;
+    int*         red_out_count;
+    const float* quant_parts;
+    const int*   quant_count;
     uint16_t*    quant_out;
-    const float* quant_max;
+    float*       quant_max_out;
+    int          parts_cap;
     // backward, stored step: image the new field against the snapshot
     // (snap_in for FP32; snap16_in + *snap16_max for F16/BF16), or no imaging
     const float*    snap_in;
@@ -80,10 +98,11 @@ struct StepArgs {
     float*       illum;
     // batched phases: the first steps of a phase read p_prev/p_cur as zero
     // instead of their buffers (no reset pass between phases); a 16-bit
-    // snapshot of the forward phase's last step is taken, at the backward
-    // phase's first step, straight from pending_field (its extended field);
-    // dry runs write only p_next: no image, illumination or snapshot update
-    // (timing candidates on live state)
+    // snapshot of the forward phase's last step is imaged, at the backward
+    // phase's first step, straight from pending_field (its extended field),
+    // with the max reduced from quant_parts[0 .. *quant_count); dry runs
+    // write only p_next: no image, illumination or snapshot update (timing
+    // candidates on live state)
     int          zero_prev, zero_cur;
     const float* pending_field;
     int          dry;
@@ -138,6 +157,13 @@ inline std::vector<int> tile_widths(Family f) {
 inline int tile_of_cell(const StepGeometry& g, int ixe, int ize) {
     return ((ixe - g.x_origin) / g.tile_x) * g.ntz + (ize - g.z_origin) / g.tile_z;
 }
+
+// The two builds of the fused step kernels (step_launch.hpp): FP32 and
+// 16-bit snapshot arguments; launch_step picks one by a.snap_fmt.
+hipError_t launch_step_f32(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
+                           hipEvent_t start, hipEvent_t stop);
+hipError_t launch_step_s16(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
+                           hipEvent_t start, hipEvent_t stop);
 
 // Launch one fused step on `stream` with the geometry from step_geometry.
 // With start/stop events, they record the kernel's own start and end.

@@ -321,9 +321,9 @@ class BatchEngine:
         out = {n: getattr(s, n) for n, _ in _capi.BatchStats._fields_}
         k = s.ngroups
         names = ["lds-tile", "march", "march-dpp", "stream"]
-        out["groups"] = [{"half": s.group_half[i], "tile_x": s.group_tile_x[i],
+        out["groups"] = [{"half": s.group_half[i], "snap16": bool(s.group_snap16[i]), "tile_x": s.group_tile_x[i],
                           "family": names[s.group_family[i]], "waves_per_cu": s.group_waves_per_cu[i]}
                          for i in range(k)]
-        for n in ("group_half", "group_tile_x", "group_family", "group_waves_per_cu"):
+        for n in ("group_half", "group_tile_x", "group_family", "group_waves_per_cu", "group_snap16"):
             del out[n]
         return out

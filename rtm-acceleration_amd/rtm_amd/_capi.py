@@ -74,9 +74,9 @@ class BatchStats(C.Structure):
                 ("rebuilds", C.c_longlong), ("cell_steps", C.c_double), ("run_seconds", C.c_double),
                 ("admit_seconds", C.c_double), ("tune_seconds", C.c_double), ("retire_seconds", C.c_double),
                 ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
-                ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 6),
-                ("group_tile_x", C.c_int * 6), ("group_family", C.c_int * 6),
-                ("group_waves_per_cu", C.c_int * 6)]
+                ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 12),
+                ("group_tile_x", C.c_int * 12), ("group_family", C.c_int * 12),
+                ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

@@ -147,8 +147,9 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
                         r.image, r.illum = img, ill
         makespan = time.perf_counter() - t0
         st = b.stats()
-    return summary(requests, makespan, specs, extra={"ticks": st["ticks"], "rebuilds": st["rebuilds"],
-                                                     "groups": st["groups"]})
+    extra = {k: st[k] for k in ("ticks", "rebuilds", "tunes", "groups")}
+    extra.update({k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds", "retire_seconds")})
+    return summary(requests, makespan, specs, extra=extra)
 
 
 def serial_plain(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,

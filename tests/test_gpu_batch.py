@@ -78,7 +78,9 @@ def test_batch_equals_alone(jobs, alone, use_graph, max_active):
         b.run()
         st = b.stats()
         assert st["jobs_done"] == len(jobs) and st["jobs_active"] == 0 and st["jobs_queued"] == 0
-        longest = max(2 * j[2] for j in jobs)
+        # 2 nt steps, plus one bookkeeping tick between the phases of a job
+        # with 16-bit snapshots
+        longest = max(2 * j[2] + (j[5] != 0) for j in jobs)
         if max_active == 0:
             assert st["ticks"] == longest          # every job ran from the first tick
         else:
@@ -237,6 +239,26 @@ def test_single_short_job_after_autotune(jobs, alone, use_graph):
             I, S = b.result(jid)
         assert bits_equal(alone[4][0], I)
         assert bits_equal(alone[4][1], S)
+
+
+def test_retune_on_load_class_change(jobs, alone):
+    """A job joining a lone order-8 job doubles the group's cells (a new load
+    class): the group is re-tuned mid-stream on dry steps, and when the
+    joiner retires the first class's cached choice comes back without a
+    search; both images keep the plain engine's bits."""
+    import rtm_amd
+    print("case: jobs 0 and 3 (order 8), job 3 joining at tick 40", flush=True)
+    with rtm_amd.BatchEngine(0) as b:
+        j0 = b.submit(*jobs[0][:5], snapshot_format=jobs[0][5])
+        b.run(max_ticks=40, until_done=False)
+        assert b.stats()["tunes"] == 1
+        j3 = b.submit(*jobs[3][:5], snapshot_format=jobs[3][5])
+        b.run()
+        assert b.stats()["tunes"] == 2
+        for k, jid in ((0, j0), (3, j3)):
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}"
+            assert bits_equal(alone[k][1], S), f"job {k}"
 
 
 def test_reserved_arena(jobs, alone):
