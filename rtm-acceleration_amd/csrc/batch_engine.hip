@@ -87,6 +87,7 @@ struct Job {
     float sx = 0.0f, sz = 0.0f;
     std::vector<float> rx, rz, traces;
     int trace_nt = 0;
+    float* d_traces = nullptr;       // device copy of the traces (job allocation)
     int stack_ix0 = -1, stack_iz0 = -1;
     size_t bytes = 0;                // device bytes of the job itself (model excluded)
     std::vector<void*> allocs;       // freed when the job retires
@@ -293,6 +294,7 @@ size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
         bytes += ((size_t)m.nsnap + 2 * tiles_bound + 2 * rtm_hip::kReduceWaves + 5) * sizeof(float);
     bytes += 2 * m.n_int * sizeof(float);                   // image, illumination
     bytes += ntraces * sizeof(float);                       // receiver data
+    bytes += (size_t)m.nt * nrec * sizeof(float);           // ... in backward-entry order
     bytes += 2 * (tiles_bound * sizeof(int) + (nrec + 1) * sizeof(Entry));
     return bytes;
 }
@@ -400,6 +402,10 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
         j.desc.tile_ptr[k] = d_ptr;
         j.desc.entries[k] = d_ent;
     }
+    // the receiver amplitudes in the new backward-entry order
+    hipError_t e = rtm_hip::launch_gather_amps(j.desc.entries[1], j.desc.amp_count, j.d_traces, j.trace_nt, m.nt,
+                                               j.desc.amps, b->stream);
+    if (e != hipSuccess) return b->hip_fail(e, "amplitude table");
     j.desc.ntz = g.ntz;
     j.desc.ntx = g.ntx;
     j.desc.ntiles = g.ntiles;
@@ -417,12 +423,13 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     d.nxe = m.nxe; d.nze = m.nze; d.ldz = m.ldz; d.nb = m.params.nb; d.nx = m.grid.nx; d.nz = m.grid.nz;
     d.half = m.half; d.nt = m.nt; d.store = m.params.store_interval; d.snap_fmt = m.params.snapshot_format;
     d.coef = m.coef;
-    d.trace_stride = j.trace_nt;
     d.n_int = (long long)m.n_int;
     d.bwd_base = (m.nt + 1) % 3;            // buffer of the forward phase's last field
     d.vdt2 = m.vdt2; d.wx = m.wx; d.wz = m.wz; d.wavelet = m.wavelet;
     float *tr = nullptr, *img = nullptr, *ill = nullptr;
     int rc = job_alloc(b, j.allocs, &tr, j.traces.size(), "traces");
+    float* amps = nullptr;
+    if (!rc) rc = job_alloc(b, j.allocs, &amps, (size_t)m.nt * j.rx.size(), "amplitude table");
     if (!rc) rc = job_alloc(b, j.allocs, &img, m.n_int, "image");
     if (!rc) rc = job_alloc(b, j.allocs, &ill, m.n_int, "illumination");
     for (int k = 0; k < 3 && !rc; ++k) rc = job_alloc(b, j.allocs, &d.fld[k], m.n_fld, "wavefield");
@@ -445,7 +452,10 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     if (e == hipSuccess && d.snap_max) e = hipMemset(d.snap_max, 0, (size_t)m.nsnap * sizeof(float));
     if (e == hipSuccess && d.part_counts) e = hipMemset(d.part_counts, 0, 4 * sizeof(int));
     if (e != hipSuccess) return b->hip_fail(e, "job upload");
-    d.traces = tr; d.image = img; d.illum = ill;
+    j.d_traces = tr;
+    d.amps = amps;
+    d.amp_count = (int)j.rx.size();
+    d.image = img; d.illum = ill;
     d.steps = 2 * m.nt + (d.snap_fmt == RTM_HIP_SNAP_F32 ? 0 : 1);     // + the bookkeeping tick
     j.steps_left = d.steps;
     j.traces.clear();

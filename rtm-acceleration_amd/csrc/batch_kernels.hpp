@@ -28,13 +28,13 @@ struct JobDesc {
     int steps;                   // forward nt, then (16-bit snapshots) one
                                  // bookkeeping tick, then backward nt
     Coeffs coef;
-    long long trace_stride;
     long long n_int;
     const float* vdt2;
     const float* wx;
     const float* wz;
     const float* wavelet;
-    const float* traces;
+    float* amps;                 // receiver amplitudes in backward-entry order (StepArgs::amps)
+    int amp_count;
     float* fld[3];
     float* snap32;               // FP32 snapshots, or
     uint16_t* snap16;            // 16-bit codes +

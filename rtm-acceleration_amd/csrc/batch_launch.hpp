@@ -44,8 +44,8 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dr
     a.tile_ptr = fwd ? J.tile_ptr[0] : J.tile_ptr[1];
     a.entries = fwd ? J.entries[0] : J.entries[1];
     a.wavelet = J.wavelet;
-    a.traces = J.traces;
-    a.trace_stride = J.trace_stride;
+    a.amps = J.amps;
+    a.amp_count = J.amp_count;
     a.image = J.image;
     a.illum = J.illum;
     a.dry = dry;

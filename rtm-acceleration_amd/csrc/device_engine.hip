@@ -87,6 +87,8 @@ struct rtm_hip_engine {
     size_t ent_cap[2] = {0, 0};
     float* d_traces = nullptr;
     size_t traces_cap = 0;
+    float* d_amps = nullptr;         // receiver amplitudes in backward-entry order (StepArgs::amps)
+    size_t amps_cap = 0;
     float* d_rec = nullptr;
     size_t rec_cap = 0;
     size_t device_bytes = 0;
@@ -162,13 +164,13 @@ struct rtm_hip_engine {
         };
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
-        f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_slice_max); f(d_part_counts); f(d_image); f(d_illum); f(d_traces); f(d_rec);
+        f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_slice_max); f(d_part_counts); f(d_image); f(d_illum); f(d_traces); f(d_amps); f(d_rec);
         f(d_stack_i); f(d_stack_s);
         for (int k = 0; k < 2; ++k) {
             f(d_tile_ptr[k]); f(d_ent[k]);
             tile_ptr_cap[k] = ent_cap[k] = 0;
         }
-        traces_cap = rec_cap = 0;
+        traces_cap = amps_cap = rec_cap = 0;
         device_bytes = 0;
         for (auto& ev : ev_pool) {
             (void)hipEventDestroy(ev.start);
@@ -395,8 +397,8 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     a.tile_ptr = h->d_tile_ptr[k];
     a.entries = h->d_ent[k];
     a.wavelet = h->d_wavelet;
-    a.traces = h->d_traces;
-    a.trace_stride = s->nt;
+    a.amps = h->d_amps;
+    a.amp_count = (int)h->h_ent[1].size();
     a.rec_out = record ? h->d_rec : nullptr;
     a.rec_stride = h->nt;
     a.image = h->d_image;
@@ -656,6 +658,10 @@ int backward_impl(rtm_hip_engine* h, const rtm_hip_shot* s) {
         hipError_t e = hipMemcpyAsync(h->d_traces, s->traces, n * sizeof(float), hipMemcpyHostToDevice,
                                       h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "upload of traces");
+        const int nent = (int)h->h_ent[1].size();
+        if (int rc = h->grow(&h->d_amps, &h->amps_cap, (size_t)h->nt * nent, "amplitude table")) return rc;
+        e = rtm_hip::launch_gather_amps(h->d_ent[1], nent, h->d_traces, s->nt, h->nt, h->d_amps, h->stream);
+        if (e != hipSuccess) return h->hip_fail(e, "amplitude table");
         e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "upload of traces");
         h->times.h2d += t.elapsed();

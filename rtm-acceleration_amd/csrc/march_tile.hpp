@@ -107,6 +107,13 @@ struct TileEntries {
     float amp;
 };
 
+// tile_ptr[t] (t is the wave's tile, the same in every lane) with a scalar
+// load: no wait on the vector loads in flight, and the entry loops run on
+// scalar registers, with scalar loads for their rare entries past 64.
+__device__ __forceinline__ int entry_bound(const StepArgs& a, int t) {
+    return sld(a.tile_ptr + __builtin_amdgcn_readfirstlane(t));
+}
+
 template <Dir D>
 __device__ __forceinline__ TileEntries load_entries(const StepArgs& a, int e0, int e1, int lane) {
     TileEntries te;
@@ -114,18 +121,27 @@ __device__ __forceinline__ TileEntries load_entries(const StepArgs& a, int e0, i
     te.cell = -1;
     te.code = 0;
     te.amp = 0.0f;
+    // No load here depends on another: the forward amplitude is the wavelet
+    // sample for every entry (only source entries, code < 0, use it).
+    if constexpr (D == Dir::Forward) te.amp = e0 < e1 ? sld(a.wavelet + a.it) : 0.0f;
     if (e0 + lane < e1) {
         te.cell = gbl(a.entries)[e0 + lane].cell;
         te.code = gbl(a.entries)[e0 + lane].code;
-        if constexpr (D == Dir::Backward) te.amp = gbl(a.traces)[(ptrdiff_t)te.code * a.trace_stride + a.it];
-        else if (te.code < 0) te.amp = gbl(a.wavelet)[a.it];
+        if constexpr (D == Dir::Backward) te.amp = gbl(a.amps)[(ptrdiff_t)a.it * a.amp_count + e0 + lane];
     }
     return te;
 }
 
+// Waits, once and before any store of the tile, for the entry registers:
+// a first use inside the per-cell entry loops would wait there instead, on
+// a counter that by then also counts the tile's earlier stores.
+__device__ __forceinline__ void entries_ready(const TileEntries& te) {
+    asm volatile("" ::"v"(te.cell), "v"(te.code), "v"(te.amp));
+}
+
 __device__ __forceinline__ int entry_cell(const StepArgs& a, const TileEntries& te, int e) {
     const int k = e - te.base;
-    return k < kTileZ ? __builtin_amdgcn_readlane(te.cell, k) : gbl(a.entries)[e].cell;
+    return k < kTileZ ? __builtin_amdgcn_readlane(te.cell, k) : sld(&a.entries[e].cell);
 }
 
 // Source / receiver injection and trace recording of one output cell, in
@@ -143,10 +159,13 @@ __device__ __forceinline__ float apply_entries(const StepArgs& a, const TileEntr
             en_code = __builtin_amdgcn_readlane(te.code, k);
             amp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(te.amp), k));
         } else {
-            en_cell = gbl(a.entries)[e].cell;
-            en_code = gbl(a.entries)[e].code;
-            if constexpr (D == Dir::Backward) amp = gbl(a.traces)[(ptrdiff_t)en_code * a.trace_stride + a.it];
-            else amp = en_code < 0 ? gbl(a.wavelet)[a.it] : 0.0f;
+            // entries past the wave's 64 lanes: scalar loads (e is uniform),
+            // which wait on their own counter; a vector load here would make
+            // the loop wait for every store before it
+            en_cell = sld(&a.entries[e].cell);
+            en_code = sld(&a.entries[e].code);
+            if constexpr (D == Dir::Backward) amp = sld(a.amps + (ptrdiff_t)a.it * a.amp_count + e);
+            else amp = en_code < 0 ? sld(a.wavelet + a.it) : 0.0f;
         }
         if (en_cell != cell) continue;
         if constexpr (D == Dir::Forward) {
@@ -439,9 +458,10 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
 #pragma unroll
     for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D, S16>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
-    const int e0 = sld(a.tile_ptr + t), e1 = sld(a.tile_ptr + t + 1);
+    const int e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
     snap_scales<S16>(a, sc, lane);
+    entries_ready(te);
     float vmax = 0.0f;
 
 #pragma unroll
@@ -488,10 +508,11 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
 #pragma unroll
     for (int r = 0; r < C; ++r) ops[r] = column_ops<H, D, S16>(a, sc, x0 + r, z, z_out, z_int, jz);
     const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
-    int ec = sld(a.tile_ptr + t);
-    const int e1 = sld(a.tile_ptr + t + 1);
+    int ec = entry_bound(a, t);
+    const int e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, ec, e1, lane);
     snap_scales<S16>(a, sc, lane);
+    entries_ready(te);
     float vmax = 0.0f;
 
 #pragma unroll

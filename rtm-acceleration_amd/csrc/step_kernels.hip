@@ -74,6 +74,16 @@ namespace {
 
 using namespace device;
 
+// Receiver amplitudes in backward-entry order, time-major (StepArgs::amps).
+// Threads run along entries, so the writes are contiguous.
+__global__ __launch_bounds__(256) void k_gather_amps(const Entry* ent, int nent, const float* traces,
+                                                     long long stride, int nt, float* amps) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)nt * nent) return;
+    const int it = (int)(k / nent), e = (int)(k % nent);
+    amps[k] = traces[(long long)ent[e].code * stride + it];
+}
+
 // Quantise the interior of an extended field into a 16-bit snapshot.
 __global__ __launch_bounds__(256) void k_quantise(const float* p, int ldz, int nb, int nx, int nz, int fmt,
                                                   const float* maxv, uint16_t* out) {
@@ -143,6 +153,15 @@ hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs&
                        hipEvent_t start, hipEvent_t stop) {
     return a.snap_fmt == (int)SnapFmt::F32 ? launch_step_f32(dir, half, g, a, stream, start, stop)
                                            : launch_step_s16(dir, half, g, a, stream, start, stop);
+}
+
+hipError_t launch_gather_amps(const Entry* ent, int nent, const float* traces, long long trace_stride, int nt,
+                              float* amps, hipStream_t stream) {
+    const long long n = (long long)nt * nent;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_amps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ent, nent, traces,
+                       trace_stride, nt, amps);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_max(const float* parts, int n, float* out, hipStream_t stream) {

@@ -28,7 +28,7 @@ struct Coeffs {
 // An injection/recording entry of the per-tile table: the extended-grid cell
 // (device index ix*ldz+iz) and a code: -1 = add vdt2*wavelet[it] (source),
 // >= 0 = receiver ir (forward: record the cell into rec_out[ir*rec_stride+it];
-// backward: add vdt2*traces[ir*trace_stride+it]).
+// backward: add vdt2 times receiver ir's sample at it).
 struct Entry {
     int cell;
     int code;
@@ -57,8 +57,12 @@ struct StepArgs {
     const int*   tile_ptr;    // [ntiles+1]
     const Entry* entries;
     const float* wavelet;     // [nt]   forward source amplitude
-    const float* traces;      // backward: trace-major receiver data
-    long long    trace_stride;
+    // backward: receiver amplitudes in entry order, time-major:
+    // amps[it * amp_count + e] is the amplitude of entry e at time it
+    // (launch_gather_amps), so a wave loads its entries' amplitudes with
+    // the entries themselves, not after them
+    const float* amps;
+    int          amp_count;
     float*       rec_out;     // forward modelling: trace-major, stride rec_stride
     long long    rec_stride;
     int          snap_fmt;    // SnapFmt
@@ -169,6 +173,11 @@ hipError_t launch_step_s16(Dir dir, int half, const StepGeometry& g, const StepA
 // With start/stop events, they record the kernel's own start and end.
 hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs& a, hipStream_t stream,
                        hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+
+// amps[it * nent + e] = traces[ent[e].code * trace_stride + it] for it < nt,
+// e < nent: the backward entry table's receiver amplitudes in entry order.
+hipError_t launch_gather_amps(const Entry* ent, int nent, const float* traces, long long trace_stride, int nt,
+                              float* amps, hipStream_t stream);
 
 // Quantise the interior of field p (extended layout) into a 16-bit snapshot
 // with the scale of *max (for a stored step that has no following step).

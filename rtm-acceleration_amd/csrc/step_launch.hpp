@@ -79,9 +79,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     for (int j = 0; j < RX + 2 * H; ++j) q[j] = tile[(lx0 + j) * LZ + H + lane];
 
     const float wzv = a.wz[iz];
-    const int e0 = sld(a.tile_ptr + t), e1 = sld(a.tile_ptr + t + 1);
+    const int e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
     snap_scales<S16>(a, sc, lane);
+    entries_ready(te);
     float vmax = 0.0f;
 
 #pragma unroll
