@@ -51,10 +51,24 @@ __device__ __forceinline__ float from_lane_below(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+// max over the wave's 64 lanes, in every lane.  DPP moves (no LDS round
+// trips): a running max along each row of 16 lanes (row_shr 1, 2, 4, 8; a lane
+// with no source keeps its own value), then row_bcast:15 and row_bcast:31
+// carry the row maxima up to lane 63, which is read back.  max is exact, so
+// the order does not change the result.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_max(float v) {
+    const int o = __float_as_int(v);
+    return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(o, o, kCtrl, kRowMask, 0xf, false)));
+}
 __device__ __forceinline__ float wave_max(float m) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, kTileZ));
-    return m;
+    m = dpp_max<0x111, 0xf>(m);      // row_shr:1
+    m = dpp_max<0x112, 0xf>(m);      // row_shr:2
+    m = dpp_max<0x114, 0xf>(m);      // row_shr:4
+    m = dpp_max<0x118, 0xf>(m);      // row_shr:8
+    m = dpp_max<0x142, 0xa>(m);      // row_bcast:15 into rows 1 and 3
+    m = dpp_max<0x143, 0xc>(m);      // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 63));
 }
 
 // max(0, parts[0..n)) over a workgroup of kThreads threads, in thread 0.  Each
