@@ -107,8 +107,8 @@ __device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int la
         for (int r = 0; r < TX; ++r) {
             const int x = H + tx * TX + r, jx = x - J.nb;
             if (z_int && x < J.nxe - H && jx >= 0 && jx < J.nx)
-                gbl(out)[(ptrdiff_t)jx * J.nz + jz] =
-                    encode_snapshot(gbl(f)[(ptrdiff_t)x * J.ldz + z], k, J.snap_fmt);
+                __builtin_nontemporal_store(encode_snapshot(gbl(f)[(ptrdiff_t)x * J.ldz + z], k, J.snap_fmt),
+                                            gbl(out) + (ptrdiff_t)jx * J.nz + jz);
         }
     }
     if (((nt - 1) % st) == 0) {

@@ -256,9 +256,9 @@ __device__ __forceinline__ void snap_scales(const StepArgs& a, SnapCtx& c, int l
 // wait for this load, and with it for every load before it).
 template <bool S16>
 __device__ __forceinline__ uint32_t snapshot_bits(const StepArgs& a, ptrdiff_t j, ptrdiff_t i) {
-    if constexpr (!S16) return __float_as_uint(gbl(a.snap_in)[j]);
+    if constexpr (!S16) return __float_as_uint(__builtin_nontemporal_load(gbl(a.snap_in) + j));
     const uintptr_t addr = a.pending_field ? (uintptr_t)(a.pending_field + i) : (uintptr_t)(a.snap16_in + j);
-    return *gbl((const uint32_t*)(addr & ~(uintptr_t)3));
+    return __builtin_nontemporal_load(gbl((const uint32_t*)(addr & ~(uintptr_t)3)));
 }
 
 // 16-bit code -> the stored float (exact): the hardware conversion for FP16,
@@ -353,6 +353,10 @@ __device__ __forceinline__ float load_cur(const StepArgs& a, int x, int z, bool 
     return (z_load && x < a.nxe && !a.zero_cur) ? gbl(a.p_cur)[(ptrdiff_t)x * a.ldz + z] : 0.0f;
 }
 
+// Snapshots are written once and read once, thousands of steps apart: their
+// stores and loads are non-temporal, so they do not displace the fields,
+// tables and image that every step re-reads from the caches.
+//
 // Leapfrog update, damping, injection and the snapshot / imaging epilogue of
 // one output cell (x, z) (src/rtm/forward.cpp:45, :56-63, :112-118;
 // src/rtm/backward.cpp:57-75):
@@ -379,9 +383,10 @@ __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc
         if constexpr (D == Dir::Forward) {
             if constexpr (S16) {
                 if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-                if (sc.quantise && !a.dry) gbl(a.quant_out)[j] = encode_snapshot(o.pv, sc.k_q, a.snap_fmt);
+                if (sc.quantise && !a.dry)
+                    __builtin_nontemporal_store(encode_snapshot(o.pv, sc.k_q, a.snap_fmt), gbl(a.quant_out) + j);
             } else {
-                if (sc.store32 && !a.dry) gbl(a.snap_out)[j] = out;
+                if (sc.store32 && !a.dry) __builtin_nontemporal_store(out, gbl(a.snap_out) + j);
             }
         } else {
             if (sc.imaging && !a.dry) {

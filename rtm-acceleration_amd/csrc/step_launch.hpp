@@ -114,9 +114,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
             if constexpr (D == Dir::Forward) {
                 if constexpr (S16) {
                     if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-                    if (sc.quantise && !a.dry) a.quant_out[j] = encode_snapshot(pv[r], sc.k_q, a.snap_fmt);
+                    if (sc.quantise && !a.dry)
+                        __builtin_nontemporal_store(encode_snapshot(pv[r], sc.k_q, a.snap_fmt), a.quant_out + j);
                 } else {
-                    if (sc.store32 && !a.dry) a.snap_out[j] = out;
+                    if (sc.store32 && !a.dry) __builtin_nontemporal_store(out, a.snap_out + j);
                 }
             } else {
                 if (sc.imaging && !a.dry) {
