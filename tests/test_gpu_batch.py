@@ -261,6 +261,55 @@ def test_retune_on_load_class_change(jobs, alone):
             assert bits_equal(alone[k][1], S), f"job {k}"
 
 
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_dense_receivers_in_batch(hip_engine_cls, jobs, fmt):
+    """More than 64 receiver entries in one tile (a 12 x 12 patch, some
+    sharing cells) beside an ordinary job: entries past the first 64 of a
+    tile, and their amplitudes in entry order, keep the plain engine's bits."""
+    import rtm_amd
+    print(f"case: dense receivers in a batch, snapshot format {fmt}", flush=True)
+    model = small_model(61, 47)
+    params = rtm_amd.RTMParams(nb=12, order=8, store_interval=2, f0=15.0)
+    nt, dt = 140, 0.001
+    gx, gz = np.meshgrid(np.arange(12) * 10.0 + 150.0, np.arange(12) * 10.0 + 60.0, indexing="ij")
+    rx = np.concatenate([gx.ravel(), [150.0, 151.0, 260.0]]).astype(np.float32)
+    rz = np.concatenate([gz.ravel(), [60.0, 61.0, 170.0]]).astype(np.float32)
+    shot = rtm_amd.ShotRecord(300.0, 20.0, rx, rz, nt + 5, dt)
+    rng = np.random.default_rng(7)
+    shot.traces = (rng.standard_normal((len(rx), nt + 5)) * 1e-2).astype(np.float32)
+    I0, S0 = _alone(hip_engine_cls, model, params, nt, dt, shot, fmt)
+    with rtm_amd.BatchEngine(0) as b:
+        other = b.submit(*jobs[0][:5], snapshot_format=jobs[0][5])
+        jid = b.submit(model, params, nt, dt, shot, snapshot_format=fmt)
+        b.run()
+        I, S = b.result(jid)
+        b.result(other)
+    assert bits_equal(I0, I) and bits_equal(S0, S)
+
+
+@pytest.mark.parametrize("nt,store", [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (4, 3)])
+def test_short_16bit_jobs(hip_engine_cls, nt, store):
+    """16-bit jobs whose last stored steps fall on the forward phase's last
+    two steps, including nt = 1 and 2 (the bookkeeping tick settles every
+    scale), equal the plain engine alone, with and without a longer job."""
+    import rtm_amd
+    print(f"case: short 16-bit job nt={nt} store={store}", flush=True)
+    model, params, _, dt, shot, _ = _job(3, 40, 30, 8, 10, nt, store, 1, 2)
+    I0, S0 = _alone(hip_engine_cls, model, params, nt, dt, shot, 1)
+    for with_other in (False, True):
+        with rtm_amd.BatchEngine(0) as b:
+            if with_other:
+                b.submit(*_other_job_args(), snapshot_format=1)
+            jid = b.submit(model, params, nt, dt, shot, snapshot_format=1)
+            b.run()
+            I, S = b.result(jid)
+        assert bits_equal(I0, I) and bits_equal(S0, S), f"with_other={with_other}"
+
+
+def _other_job_args():
+    return _job(9, 52, 47, 8, 16, 121, 4, 1, 0)[:5]
+
+
 def test_reserved_arena(jobs, alone):
     """Jobs placed in a reserved block: a block that holds only a few jobs at
     a time makes later jobs wait for memory (and for a free run of it), and
