@@ -340,7 +340,7 @@ def main():
     ap.add_argument("--batched-shots", type=int, default=8,
                     help="shots the batch engine migrates at once in the plain workload's comparison")
     ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
-    ap.add_argument("--rates", default="2,8,32,64", help="arrival rates (1/s) of the served workload")
+    ap.add_argument("--rates", default="1,4,16,64", help="arrival rates (1/s) of the served workload")
     ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
     ap.add_argument("--no-plain-serial", action="store_true", help="served workload without the plain baseline")
     ap.add_argument("--serve-chunk-ms", type=float, default=10.0,

@@ -275,6 +275,8 @@ typedef struct {
     int       group_waves_per_cu[12];                             /* 0: no cap */
     int       group_snap16[12];  /* 1: the group's jobs store 16-bit snapshots     */
     int       tunes;             /* autotune searches (one per group and load class) */
+    double    rebuild_seconds;   /* launch-table rebuilds (inside run_seconds)      */
+    double    capture_seconds;   /* hipGraph capture and instantiation (inside run) */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

@@ -143,6 +143,7 @@ struct rtm_hip_batch {
     std::deque<Job*> queue;
     std::vector<Job*> active;
     JobDesc* d_jobs = nullptr;
+    rtm_hip::TickArgs* d_targs = nullptr;            // [jobs_cap][kRunTicks] (launch_tick_args)
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by group_key (half-width, snapshot width)
     bool dirty = true;
@@ -468,14 +469,27 @@ int admit_job(rtm_hip_batch* b, Job& j) {
 // The launch tables of every half-width group and the device descriptors.
 // Ticks still running on the stream read the descriptors, the step arguments
 // and the tables rewritten here, so the stream is drained first.
+int rebuild_tables(rtm_hip_batch* b);
+
 int rebuild(rtm_hip_batch* b) {
+    const double t0 = now_s();
+    const int rc = rebuild_tables(b);
+    b->stats.rebuild_seconds += now_s() - t0;
+    return rc;
+}
+
+int rebuild_tables(rtm_hip_batch* b) {
     if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess) return b->hip_fail(e, "batch synchronisation");
     b->drop_graph();
     if (b->active.size() > b->jobs_cap) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
+        if (b->d_targs) (void)hipFree(b->d_targs);
         b->d_jobs = nullptr;
+        b->d_targs = nullptr;
         b->jobs_cap = std::max<size_t>(b->active.size(), 2 * b->jobs_cap);
         hipError_t e = hipMalloc(&b->d_jobs, b->jobs_cap * sizeof(JobDesc));
+        if (e == hipSuccess)
+            e = hipMalloc(&b->d_targs, b->jobs_cap * rtm_hip::kRunTicks * sizeof(rtm_hip::TickArgs));
         if (e != hipSuccess) {
             b->jobs_cap = 0;
             return b->hip_fail(e, "job descriptors");
@@ -535,14 +549,19 @@ int rebuild(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
-// A run of `ticks` ticks: the group launches of every tick, then one clock
-// update for the run.
+// A run of `ticks` ticks (at most kRunTicks): the tick arguments of the run,
+// the group launches of every tick, then one clock update for the run.
 int enqueue_ticks(rtm_hip_batch* b, int ticks) {
+    if (ticks < 1 || ticks > rtm_hip::kRunTicks)
+        return b->fail(RTM_HIP_ERR_ARG, "internal: a run of %d ticks (1..%d)", ticks, rtm_hip::kRunTicks);
+    hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, ticks, 0, b->stream);
+    if (ea != hipSuccess) return b->hip_fail(ea, "tick arguments");
     for (int sub = 0; sub < ticks; ++sub) {
         for (auto& [key, g] : b->groups) {
             if (g.ntiles == 0) continue;
             hipError_t e = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key), b->d_jobs,
-                                                      g.d_tile_job, g.ntiles, sub, 0, g.lds_bytes, b->stream);
+                                                      b->d_targs, g.d_tile_job, g.ntiles, sub, g.lds_bytes,
+                                                      b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
         }
     }
@@ -575,12 +594,18 @@ int autotune(rtm_hip_batch* b) {
     // 2 warm dry steps, then the best of `rounds` rounds of `reps` timed dry steps
     auto time_group = [&](int key, const Group& g, int rounds, int reps) -> float {
         float ms = 1e30f;
+        // dry tick arguments for tick 0 of the current clocks (the next run
+        // rewrites the table)
+        if (hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, 1, 1,
+                                                      b->stream);
+            ea != hipSuccess)
+            rc = b->hip_fail(ea, "tick arguments");
         for (int round = 0; round < rounds && !rc; ++round) {
             for (int rep = 0; rep < (round == 0 ? reps + 2 : reps) && !rc; ++rep) {
                 if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
                 hipError_t le = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key),
-                                                           b->d_jobs, g.d_tile_job, g.ntiles, 0, 1, g.lds_bytes,
-                                                           b->stream);
+                                                           b->d_jobs, b->d_targs, g.d_tile_job, g.ntiles, 0,
+                                                           g.lds_bytes, b->stream);
                 if (le != hipSuccess) rc = b->hip_fail(le, "autotune launch");
             }
             if (rc) break;
@@ -732,7 +757,16 @@ int retire_finished(rtm_hip_batch* b) {
 }
 
 // Captures `ticks` consecutive ticks into graph slot k.
+int capture_ticks_now(rtm_hip_batch* b, int k, int ticks);
+
 int capture_ticks(rtm_hip_batch* b, int k, int ticks) {
+    const double t0 = now_s();
+    const int rc = capture_ticks_now(b, k, ticks);
+    b->stats.capture_seconds += now_s() - t0;
+    return rc;
+}
+
+int capture_ticks_now(rtm_hip_batch* b, int k, int ticks) {
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return b->hip_fail(e, "graph capture");
     const int rc = enqueue_ticks(b, ticks);
@@ -755,7 +789,7 @@ int capture_ticks(rtm_hip_batch* b, int k, int ticks) {
 // Ticks between job-set changes replay captured graphs: kGraphTicks ticks per
 // launch while that many remain (one host call for many small ticks), then
 // single ticks.
-constexpr int kGraphTicks = 32;
+constexpr int kGraphTicks = rtm_hip::kRunTicks;
 
 int run_ticks(rtm_hip_batch* b, int n) {
     if (b->cfg.use_graph) {
@@ -867,6 +901,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     for (auto& [key, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_targs) (void)hipFree(b->d_targs);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
     if (b->arena.active()) (void)hipFree(b->arena.base());

@@ -16,9 +16,30 @@
 // clock and stack kernels and picks the build.
 #include "batch_kernels.hpp"
 
+#include "batch_launch.hpp"
+
 namespace rtm_hip {
 
 namespace {
+
+// Tick arguments of ticks 0..nsub-1 of the coming run, one thread per job
+// and tick.
+__global__ __launch_bounds__(64) void k_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int nsub,
+                                                  int dry) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= njobs * nsub) return;
+    const int jid = i / nsub, sub = i % nsub;
+    const JobDesc& J = jobs[jid];
+    const int total = J.done + sub;
+    const bool s16 = J.snap_fmt != (int)SnapFmt::F32;
+    TickArgs& T = targs[jid * kRunTicks + sub];
+    T.kind = total >= J.steps ? kTickIdle
+             : total < J.nt  ? kTickForward
+             : (s16 && total == J.nt) ? kTickBookkeeping
+                                      : kTickBackward;
+    T.tile_first = J.tile_first;
+    T.a = s16 ? job_args<true>(J, total, dry) : job_args<false>(J, total, dry);
+}
 
 // Job clocks: every job has done `ticks` more steps.
 __global__ __launch_bounds__(64) void k_advance(JobDesc* jobs, int njobs, int ticks) {
@@ -39,9 +60,19 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 }  // namespace
 
 hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, const JobDesc* jobs,
-                             const int* tile_job, int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream) {
-    return snap16 ? launch_batch_step_s16(half, family, tile_x, jobs, tile_job, ntiles, sub, dry, lds_bytes, stream)
-                  : launch_batch_step_f32(half, family, tile_x, jobs, tile_job, ntiles, sub, dry, lds_bytes, stream);
+                             const TickArgs* targs, const int* tile_job, int ntiles, int sub, int lds_bytes,
+                             hipStream_t stream) {
+    return snap16 ? launch_batch_step_s16(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream)
+                  : launch_batch_step_f32(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream);
+}
+
+hipError_t launch_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int nsub, int dry,
+                            hipStream_t stream) {
+    const int n = njobs * nsub;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tick_args, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, jobs, njobs, targs, nsub,
+                       dry);
+    return hipGetLastError();
 }
 
 hipError_t launch_stack_add(const float* image, const float* illum, int nx, int nz, double* stack_i,

@@ -52,20 +52,45 @@ struct JobDesc {
     const Entry* entries[2];
 };
 
+// Ticks per captured run; the tick-argument table holds this many entries
+// per job.
+constexpr int kRunTicks = 32;
+
+// One job's arguments for one tick of a run, resolved once per run by
+// launch_tick_args: the step arguments (buffers, slots, flags of that step)
+// and what the tick is for the job.  The step kernel reads them with scalar
+// loads that do not depend on each other, instead of resolving them from
+// the descriptor in every wave (a chain of dependent scalar loads and integer
+// divisions before the first vector load).
+enum TickKind : int { kTickForward = 0, kTickBackward = 1, kTickBookkeeping = 2, kTickIdle = 3 };
+struct TickArgs {
+    StepArgs a;
+    int kind;                    // TickKind
+    int tile_first;              // the job's first tile in its group launch
+};
+
+// Tick arguments of ticks 0..nsub-1 of the coming run for every job:
+// targs[job * kRunTicks + sub] (nsub <= kRunTicks).  dry != 0 marks them as
+// timing launches (StepArgs::dry).
+hipError_t launch_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int nsub, int dry,
+                            hipStream_t stream);
+
 // Tick `sub` of a run for the jobs of one half-width H: tile_job[g] names
-// the job of global tile g, and job j computes its step jobs[j].done + sub;
-// family kMarch, kMarchDpp or kStream with the tile widths of the per-shot
-// kernels.  dry != 0 writes only p^{n+1}: no image, illumination or snapshot
+// the job of global tile g, and job j computes the step described by
+// targs[j * kRunTicks + sub] (launch_tick_args); family kMarch, kMarchDpp or
+// kStream with the tile widths of the per-shot kernels.  Tick arguments made
+// with dry != 0 write only p^{n+1}: no image, illumination or snapshot
 // update (used to time candidate kernels on live state).  lds_bytes of
 // dynamic LDS per workgroup cap the workgroups resident on a CU.
 // snap16 selects the build for jobs with 16-bit snapshots (every job of
 // the launch must have the same snapshot width).
 hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, const JobDesc* jobs,
-                             const int* tile_job, int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
-hipError_t launch_batch_step_f32(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                                 int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
-hipError_t launch_batch_step_s16(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                                 int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream);
+                             const TickArgs* targs, const int* tile_job, int ntiles, int sub, int lds_bytes,
+                             hipStream_t stream);
+hipError_t launch_batch_step_f32(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
+                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream);
+hipError_t launch_batch_step_s16(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
+                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.

@@ -19,9 +19,7 @@ __device__ __forceinline__ float* rotate(const JobDesc& J, int r) {
 
 // Step arguments of job J after `total` of its steps (forward steps
 // 0..nt-1, the bookkeeping tick of a 16-bit job, then backward steps
-// 0..nt-1).  The stencil coefficients are read from the descriptor where they
-// are used (an array copied into a local struct would be kept in scratch
-// memory).
+// 0..nt-1); written to the tick-argument table by k_tick_args.
 template <bool S16>
 __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dry) {
     StepArgs a{};
@@ -34,6 +32,7 @@ __device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dr
     a.p_next = rotate(J, (base + s + 2) % 3);
     a.zero_prev = s <= 1;
     a.zero_cur = s == 0;
+    a.coef = J.coef;
     a.vdt2 = J.vdt2;
     a.wx = J.wx;
     a.wz = J.wz;
@@ -118,47 +117,46 @@ __device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int la
     }
 }
 
-// Tick `sub` of a run of ticks for every job of half-width H: job J is at
-// step J.done + sub.  The job and tile of a wave are wave-uniform
-// (read-first-lane), so the descriptor is read with scalar loads and the
-// step arguments are built in scalar registers, as a kernel argument would
-// be.
+// Tick `sub` of a run of ticks for every job of one group: the wave's job
+// (wave-uniform, read-first-lane) and that job's tick arguments, read with
+// independent scalar loads; then the same march/stream tile as the per-shot
+// kernel.
 template <int H, int TX, Family F, bool S16>
 __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
-                                                         const int* __restrict__ tile_job, int ntiles, int sub,
-                                                         int dry) {
+                                                         const TickArgs* __restrict__ targs,
+                                                         const int* __restrict__ tile_job, int ntiles, int sub) {
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (ntiles + kWaves - 1) / kWaves;
     const int g = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (g >= ntiles) return;
     const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
-    const JobDesc& J = jobs[jid];
-    const int total = J.done + sub;
-    if (total >= J.steps) return;            // finished job: its tiles idle
-    if (S16 && total == J.nt) {
-        if (!dry) bookkeeping_tile<H, TX>(J, g - J.tile_first, lane);
+    const TickArgs& T = targs[jid * kRunTicks + sub];
+    const int kind = T.kind;
+    if (kind == kTickIdle) return;           // finished job: its tiles idle
+    const int t = g - T.tile_first;
+    const StepArgs& a = T.a;
+    if (S16 && kind == kTickBookkeeping) {
+        if (!a.dry) bookkeeping_tile<H, TX>(jobs[jid], t, lane);
         return;
     }
-    const StepArgs a = job_args<S16>(J, total, dry);
-    const int t = g - J.tile_first;
     if constexpr (F == Family::kStream) {
-        if (total < J.nt) stream_tile<H, TX, Dir::Forward, S16>(a, J.coef, t, lane);
-        else stream_tile<H, TX, Dir::Backward, S16>(a, J.coef, t, lane);
+        if (kind == kTickForward) stream_tile<H, TX, Dir::Forward, S16>(a, a.coef, t, lane);
+        else stream_tile<H, TX, Dir::Backward, S16>(a, a.coef, t, lane);
     } else {
         constexpr bool DPP = F == Family::kMarchDpp;
-        if (total < J.nt) march_tile<H, TX, Dir::Forward, DPP, S16>(a, J.coef, t, lane);
-        else march_tile<H, TX, Dir::Backward, DPP, S16>(a, J.coef, t, lane);
+        if (kind == kTickForward) march_tile<H, TX, Dir::Forward, DPP, S16>(a, a.coef, t, lane);
+        else march_tile<H, TX, Dir::Backward, DPP, S16>(a, a.coef, t, lane);
     }
 }
 
 template <int H, bool S16>
-hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* tile_job, int ntiles, int sub,
-                    int dry, size_t lds, hipStream_t s) {
+hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs, const int* tile_job,
+                    int ntiles, int sub, size_t lds, hipStream_t s) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
 #define RTM_BATCH(TX, F) \
-    hipLaunchKernelGGL((k_batch<H, TX, F, S16>), grid, block, lds, s, jobs, tile_job, ntiles, sub, dry)
+    hipLaunchKernelGGL((k_batch<H, TX, F, S16>), grid, block, lds, s, jobs, targs, tile_job, ntiles, sub)
     if (family == Family::kStream) {
         switch (tile_x) {
             case 8: RTM_BATCH(8, Family::kStream); break;
@@ -193,17 +191,17 @@ hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const int* t
 
 
 template <bool S16>
-hipError_t launch_batch_step_t(int half, Family family, int tile_x, const JobDesc* jobs, const int* tile_job,
-                             int ntiles, int sub, int dry, int lds_bytes, hipStream_t stream) {
+hipError_t launch_batch_step_t(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
+                               const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream) {
     const size_t lds = (size_t)lds_bytes;
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
-        case 2: return launch_h<2, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
-        case 3: return launch_h<3, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
-        case 4: return launch_h<4, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
-        case 5: return launch_h<5, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
-        case 6: return launch_h<6, S16>(family, tile_x, jobs, tile_job, ntiles, sub, dry, lds, stream);
+        case 1: return launch_h<1, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 2: return launch_h<2, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 3: return launch_h<3, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 4: return launch_h<4, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 5: return launch_h<5, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 6: return launch_h<6, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
         default: return hipErrorInvalidValue;
     }
 }

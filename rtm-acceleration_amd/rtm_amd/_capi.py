@@ -76,7 +76,8 @@ class BatchStats(C.Structure):
                 ("busy_latency_s", C.c_double), ("models_resident", C.c_int), ("bytes_in_use", C.c_size_t),
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 12),
                 ("group_tile_x", C.c_int * 12), ("group_family", C.c_int * 12),
-                ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int)]
+                ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int),
+                ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

@@ -148,7 +148,8 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
         makespan = time.perf_counter() - t0
         st = b.stats()
     extra = {k: st[k] for k in ("ticks", "rebuilds", "tunes", "groups")}
-    extra.update({k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds", "retire_seconds")})
+    extra.update({k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds", "retire_seconds",
+                                                  "rebuild_seconds", "capture_seconds")})
     return summary(requests, makespan, specs, extra=extra)
 
 
