@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "arena.hpp"
+#include "batch_args.hpp"
 #include "batch_kernels.hpp"
 #include "host_tables.hpp"
 #include "rtm_hip.h"
@@ -791,8 +792,49 @@ int capture_ticks_now(rtm_hip_batch* b, int k, int ticks) {
 // single ticks.
 constexpr int kGraphTicks = rtm_hip::kRunTicks;
 
+// A job alone on the device runs on the per-shot step kernels, with its
+// step arguments built on the host as the per-shot engine builds them: the
+// batch kernel's own cost, reading the wave's job and tick arguments from
+// memory, is ~1.5 us a tick, which only many jobs per launch hide.  The same
+// tile function runs either way, so the results are the same bits.  The
+// bookkeeping tick of a 16-bit job still runs on the batch kernel.
+int run_solo(rtm_hip_batch* b, int n) {
+    Job& j = *b->active.front();
+    const Model& m = *j.model;
+    const int key = group_key(m);
+    const Group& g = b->groups[key];
+    rtm_hip::StepGeometry geom = rtm_hip::step_geometry(g.family, m.half, g.tile_x, m.nxe, m.nze, m.ldz);
+    geom.blocks_per_cu = g.blocks_per_cu;
+    geom.lds_bytes = g.lds_bytes;
+    const bool s16 = key_snap16(key);
+    const int done0 = j.desc.steps - j.steps_left;
+    int synced = 0;                                  // ticks already added to the device clock
+    for (int i = 0; i < n; ++i) {
+        const int done = done0 + i;
+        hipError_t e = hipSuccess;
+        if (s16 && done == m.nt) {
+            e = rtm_hip::launch_batch_advance(b->d_jobs, 1, i - synced, b->stream);
+            synced = i;
+            if (e == hipSuccess) e = rtm_hip::launch_tick_args(b->d_jobs, 1, b->d_targs, 1, 0, b->stream);
+            if (e == hipSuccess)
+                e = rtm_hip::launch_batch_step(m.half, g.family, g.tile_x, true, b->d_jobs, b->d_targs,
+                                               g.d_tile_job, g.ntiles, 0, g.lds_bytes, b->stream);
+        } else {
+            const rtm_hip::StepArgs a = s16 ? rtm_hip::job_args<true>(j.desc, done, 0)
+                                            : rtm_hip::job_args<false>(j.desc, done, 0);
+            e = rtm_hip::launch_step(done < m.nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, m.half, geom, a,
+                                     b->stream);
+        }
+        if (e != hipSuccess) return b->hip_fail(e, "step launch");
+    }
+    hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, 1, n - synced, b->stream);
+    return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
+}
+
 int run_ticks(rtm_hip_batch* b, int n) {
-    if (b->cfg.use_graph) {
+    if (b->active.size() == 1) {
+        if (int rc = run_solo(b, n)) return rc;
+    } else if (b->cfg.use_graph) {
         int left = n;
         if (left >= kGraphTicks) {
             if (!b->graph_exec[1])

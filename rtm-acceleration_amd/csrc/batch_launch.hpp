@@ -5,6 +5,7 @@
 // snapshots run in separate launches).
 #pragma once
 
+#include "batch_args.hpp"
 #include "batch_kernels.hpp"
 #include "march_tile.hpp"
 
@@ -12,74 +13,6 @@ namespace rtm_hip {
 namespace {
 
 using namespace device;
-
-__device__ __forceinline__ float* rotate(const JobDesc& J, int r) {
-    return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
-}
-
-// Step arguments of job J after `total` of its steps (forward steps
-// 0..nt-1, the bookkeeping tick of a 16-bit job, then backward steps
-// 0..nt-1); written to the tick-argument table by k_tick_args.
-template <bool S16>
-__device__ __forceinline__ StepArgs job_args(const JobDesc& J, int total, int dry) {
-    StepArgs a{};
-    constexpr bool wide = !S16;
-    const int fwd = total < J.nt;
-    const int s = fwd ? total : total - J.nt - (wide ? 0 : 1);
-    const int base = fwd ? 0 : J.bwd_base;
-    a.p_prev = rotate(J, (base + s) % 3);
-    a.p_cur = rotate(J, (base + s + 1) % 3);
-    a.p_next = rotate(J, (base + s + 2) % 3);
-    a.zero_prev = s <= 1;
-    a.zero_cur = s == 0;
-    a.coef = J.coef;
-    a.vdt2 = J.vdt2;
-    a.wx = J.wx;
-    a.wz = J.wz;
-    a.nxe = J.nxe; a.nze = J.nze; a.ldz = J.ldz; a.nb = J.nb; a.nx = J.nx; a.nz = J.nz;
-    a.ntz = J.ntz; a.ntx = J.ntx; a.ntiles = J.ntiles;
-    const int it = fwd ? s : J.nt - 1 - s;
-    a.it = it;
-    a.tile_ptr = fwd ? J.tile_ptr[0] : J.tile_ptr[1];
-    a.entries = fwd ? J.entries[0] : J.entries[1];
-    a.wavelet = J.wavelet;
-    a.amps = J.amps;
-    a.amp_count = J.amp_count;
-    a.image = J.image;
-    a.illum = J.illum;
-    a.dry = dry;
-    a.snap_fmt = J.snap_fmt;
-    // Every field is assigned once, unconditionally: stores of different
-    // fields in different branches can be merged into one store at a
-    // computed offset, which would keep the whole struct in scratch memory.
-    const int st = J.store;
-    const bool stored = (it % st) == 0;
-    const long long slot = it / st;
-    const bool half16 = !wide && fwd;
-    const bool red = half16 && it >= 1 && ((it - 1) % st) == 0;          // slices of step it-1's maxima
-    const bool quant = half16 && it >= 2 && ((it - 2) % st) == 0;        // codes of step it-2
-    const int p0 = (int)(slot & 1), p1 = (int)(((it - 1) / st) & 1), p2 = (int)(((it - 2) / st) & 1);
-    // the last forward field, imaged at backward step 0 (its slices were
-    // reduced by the bookkeeping tick)
-    const bool pending = !wide && !fwd && stored && s == 0;
-    a.snap_out = (wide && fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
-    a.snap_in = (wide && !fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
-    a.wave_max = (half16 && stored) ? J.wave_max + (ptrdiff_t)p0 * J.parts_cap : nullptr;
-    a.wave_count = (half16 && stored) ? J.part_counts + p0 : nullptr;
-    a.red_in = red ? J.wave_max + (ptrdiff_t)p1 * J.parts_cap : nullptr;
-    a.red_count = red ? J.part_counts + p1 : nullptr;
-    a.red_out = red ? J.slice_max + p1 * kReduceWaves : nullptr;
-    a.red_out_count = red ? J.part_counts + 2 + p1 : nullptr;
-    a.quant_parts = quant ? J.slice_max + p2 * kReduceWaves : (pending ? J.slice_max + p0 * kReduceWaves : nullptr);
-    a.quant_count = quant ? J.part_counts + 2 + p2 : (pending ? J.part_counts + 2 + p0 : nullptr);
-    a.quant_out = quant ? J.snap16 + (it - 2) / st * J.n_int : nullptr;
-    a.quant_max_out = quant ? J.snap_max + (it - 2) / st : nullptr;
-    a.parts_cap = J.parts_cap;
-    a.snap16_max = (!wide && !fwd && stored && !pending) ? J.snap_max + slot : nullptr;
-    a.pending_field = pending ? rotate(J, J.bwd_base) : nullptr;
-    a.snap16_in = (!wide && !fwd && stored && !pending) ? J.snap16 + slot * J.n_int : nullptr;
-    return a;
-}
 
 // The bookkeeping tick of a 16-bit job, between its phases (wave t of the
 // job's TX-column tiles): the stored forward steps among the last two have
