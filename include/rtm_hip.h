@@ -260,7 +260,8 @@ typedef struct {
 typedef struct {
     long long jobs_submitted, jobs_done;
     int       jobs_active, jobs_queued;
-    long long ticks;             /* time steps run (each advances every active job) */
+    long long ticks;             /* ticks run (each advances every active job one step; a
+                                    job takes 2*nt ticks, plus one with 16-bit snapshots) */
     long long rebuilds;          /* launch-table rebuilds (job set changes)         */
     double    cell_steps;        /* 2*nt*nxe*nze summed over finished jobs          */
     double    run_seconds;       /* wall time inside rtm_hip_batch_run, of which: */

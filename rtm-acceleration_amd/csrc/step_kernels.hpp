@@ -179,12 +179,12 @@ hipError_t launch_step(Dir dir, int half, const StepGeometry& g, const StepArgs&
 hipError_t launch_gather_amps(const Entry* ent, int nent, const float* traces, long long trace_stride, int nt,
                               float* amps, hipStream_t stream);
 
-// Quantise the interior of field p (extended layout) into a 16-bit snapshot
-// with the scale of *max (for a stored step that has no following step).
-// *out = max(parts[0..n)) (the max of a stored 16-bit snapshot from the
-// per-wave maxima of its step); one workgroup.
+// For a 16-bit stored step among the last two of the forward loop (no step
+// it+2 settles it): *out = max(parts[0..n)) over its per-wave maxima or
+// slice maxima (one workgroup), then launch_quantise below.
 hipError_t launch_reduce_max(const float* parts, int n, float* out, hipStream_t stream);
-
+// Quantise the interior of field p (extended layout) into a 16-bit snapshot
+// with the scale of *max.
 hipError_t launch_quantise(const float* p, int ldz, int nb, int nx, int nz, SnapFmt fmt, const float* max,
                            uint16_t* out, hipStream_t stream);
 
