@@ -203,12 +203,19 @@ struct SnapCtx {
     int k_in;          // backward decode exponent
     int k_q;           // forward encode exponent
     float q_max;       // the quantised snapshot's max
+    float part;        // quant_parts[lane], loaded before the tile's loads
 };
 
-// max of parts[0 .. *count) (at most kReduceWaves values), in every lane.
-__device__ __forceinline__ float parts_max(const float* parts, const int* count, int lane) {
+// max of parts[0 .. *count) (at most kReduceWaves values), in every lane,
+// from v = parts[lane]: all kReduceWaves slots exist, so the load does not
+// wait for the count (both come from the previous launch, a memory round
+// trip each).
+__device__ __forceinline__ float parts_max_of(float v, const int* count, int lane) {
     const int n = min(sld(count), kReduceWaves);
-    return wave_max(lane < n ? gbl(parts)[lane] : 0.0f);
+    return wave_max(lane < n ? v : 0.0f);
+}
+__device__ __forceinline__ float parts_max(const float* parts, const int* count, int lane) {
+    return parts_max_of(gbl(parts)[lane], count, lane);
 }
 
 // The epilogue's flags (no memory access).
@@ -226,8 +233,17 @@ __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
         c.store_max = c.quantise = false;
     }
     c.k_in = c.k_q = 0;
-    c.q_max = 0.0f;
+    c.q_max = c.part = 0.0f;
     return c;
+}
+
+// Issues the load of this lane's slice max for the 16-bit scale of the
+// launch (quantise, or the pending snapshot's imaging) before the tile's own
+// loads: the scale's chain then overlaps them instead of following them.
+template <bool S16>
+__device__ __forceinline__ void snap_prefetch(const StepArgs& a, SnapCtx& c, int lane) {
+    if constexpr (S16)
+        if (c.quantise || (c.imaging && a.pending_field)) c.part = gbl(a.quant_parts)[lane];
 }
 
 // The 16-bit scales of the epilogue.  Called after the tile's loads are
@@ -236,10 +252,10 @@ template <bool S16>
 __device__ __forceinline__ void snap_scales(const StepArgs& a, SnapCtx& c, int lane) {
     if constexpr (S16) {
         if (c.imaging)
-            c.k_in = snapshot_scale_exponent(a.pending_field ? parts_max(a.quant_parts, a.quant_count, lane)
+            c.k_in = snapshot_scale_exponent(a.pending_field ? parts_max_of(c.part, a.quant_count, lane)
                                                              : sld(a.snap16_max));
         if (c.quantise) {
-            c.q_max = parts_max(a.quant_parts, a.quant_count, lane);
+            c.q_max = parts_max_of(c.part, a.quant_count, lane);
             c.k_q = snapshot_scale_exponent(c.q_max);
         }
     }
@@ -469,6 +485,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     const int jz = z - a.nb;                 // interior depth index
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
     SnapCtx sc = snap_ctx<S16>(a, D);
+    snap_prefetch<S16>(a, sc, lane);
 
     float q[NQ];
 #pragma unroll
@@ -519,6 +536,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
     const int jz = z - a.nb;
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
     SnapCtx sc = snap_ctx<S16>(a, D);
+    snap_prefetch<S16>(a, sc, lane);
 
     float q[W];
 #pragma unroll

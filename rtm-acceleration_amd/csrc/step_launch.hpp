@@ -32,6 +32,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int wave = (int)threadIdx.x / kTileZ;
     SnapCtx sc = snap_ctx<S16>(a, D);
+    snap_prefetch<S16>(a, sc, lane);
 
     const int iz = z0 + lane;
     const bool z_ok = iz >= H && iz < a.nze - H;
