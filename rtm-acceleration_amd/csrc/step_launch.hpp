@@ -33,6 +33,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     const int wave = (int)threadIdx.x / kTileZ;
     SnapCtx sc = snap_ctx<S16>(a, D);
     snap_prefetch<S16>(a, sc, lane);
+    const int e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);   // in flight with the tile's loads
 
     const int iz = z0 + lane;
     const bool z_ok = iz >= H && iz < a.nze - H;
@@ -80,7 +81,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     for (int j = 0; j < RX + 2 * H; ++j) q[j] = tile[(lx0 + j) * LZ + H + lane];
 
     const float wzv = a.wz[iz];
-    const int e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
     snap_scales<S16>(a, sc, lane);
     entries_ready(te);
