@@ -1,0 +1,128 @@
+"""The batched job schedule (csrc/batch_args.hpp job_args, which drives both
+the batch kernel's tick-argument table and a lone job's per-shot launches),
+checked on the host by following the data through it rather than by
+restating its formulas:
+
+* every step reads, as p^n and p^{n-1}, the buffers that the two previous
+  steps of its phase wrote, reads zero exactly where the phase has no such
+  step, and its three buffers are distinct;
+* FP32: every snapshot slot is written once in the forward phase by the
+  step that produces that field and read once in the backward phase at the
+  matching time;
+* 16-bit: the scale of slot k is reduced from the per-wave maxima that step
+  it_k wrote (same parity) and the codes are written from the field of step
+  it_k, two steps later, with the slice maxima of the same parity; the last
+  two stored steps are left to the one bookkeeping tick (or, for the last
+  field, imaged straight from the buffer that holds it); a parity buffer is
+  never written while a pending reduction still has to read it.
+"""
+import json
+import os
+import subprocess
+
+import pytest
+
+from conftest import PKG, REPO
+
+CASES = [(nt, st, fmt) for fmt in (0, 1) for nt in range(1, 10) for st in range(1, 6)] + \
+        [(40, 4, 1), (41, 3, 0), (37, 7, 1)]
+
+
+@pytest.fixture(scope="module")
+def schedules(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("sched") / "schedule_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I", os.path.join(PKG, "csrc"), "-I", os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "native", "schedule_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)] + [f"{nt}:{st}:{fmt}" for nt, st, fmt in CASES],
+                         capture_output=True, text=True, check=True).stdout
+    by_case = {}
+    for line in out.splitlines():
+        rec = json.loads(line)
+        by_case.setdefault(rec["case"], []).append(rec)
+    return by_case
+
+
+@pytest.mark.parametrize("nt,store,fmt", CASES)
+def test_schedule_dataflow(schedules, nt, store, fmt):
+    steps = schedules[f"{nt}:{store}:{fmt}"]
+    s16 = fmt != 0
+    assert [r["total"] for r in steps] == list(range(2 * nt + (1 if s16 else 0)))
+    book = [r for r in steps if r["kind"] == "book"]
+    assert len(book) == (1 if s16 else 0) and (not book or book[0]["total"] == nt)
+    fwd = [r for r in steps if r["kind"] == "fwd"]
+    bwd = [r for r in steps if r["kind"] == "bwd"]
+    assert [r["it"] for r in fwd] == list(range(nt))
+    assert [r["it"] for r in bwd] == list(range(nt - 1, -1, -1))
+    stored = [it for it in range(nt) if it % store == 0]
+    nsnap = (nt - 1) // store + 1
+    assert len(stored) == nsnap
+
+    # ---- fields: what each buffer holds, followed through both phases
+    content = {0: None, 1: None, 2: None}
+    for phase, recs in (("F", fwd), ("B", bwd)):
+        for s, r in enumerate(recs):
+            assert len({r["prev"], r["cur"], r["next"]}) == 3
+            assert r["zc"] == (s == 0) and r["zp"] == (s <= 1)
+            if s >= 1:
+                assert content[r["cur"]] == (phase, s - 1)
+            if s >= 2:
+                assert content[r["prev"]] == (phase, s - 2)
+            if phase == "B" and s == 0 and r["pending"] >= 0:
+                # the last forward field, imaged straight from its buffer
+                assert content[r["pending"]] == ("F", nt - 1)
+                assert r["next"] != r["pending"]
+            content[r["next"]] = (phase, s)
+        if phase == "F":
+            last_fwd = dict(content)
+
+    if not s16:
+        assert [r["snap_out"] for r in fwd] == [it // store if it in stored else -1 for it in range(nt)]
+        assert [r["snap_in"] for r in bwd] == [r["it"] // store if r["it"] in stored else -1 for r in bwd]
+        assert all(r["wmax"] == r["red_in"] == r["qout"] == r["s16in"] == r["pending"] == -1 for r in fwd + bwd)
+        return
+
+    # ---- 16-bit scales and codes
+    assert all(r["snap_out"] == r["snap_in"] == -1 for r in fwd + bwd)
+    by_it = {r["it"]: r for r in fwd}
+    for k, itk in enumerate(stored):
+        p = k & 1
+        assert by_it[itk]["wmax"] == p and by_it[itk]["wcount"] == p
+        if itk + 1 <= nt - 1:
+            r = by_it[itk + 1]
+            assert r["red_in"] == p and r["red_count"] == p and r["red_out"] == p and r["red_out_count"] == 2 + p
+        if itk + 2 <= nt - 1:
+            r = by_it[itk + 2]
+            assert r["qout"] == k and r["qmax"] == k and r["qparts"] == p and r["qcount"] == 2 + p
+            # the codes come from p^{n-1} of that step: the field of step it_k
+            # (checked by the dataflow above: prev holds F it_k at step it_k+2)
+    # no step reduces or quantises a slot it should not
+    reduced = [r["it"] - 1 for r in fwd if r["red_in"] >= 0]
+    assert reduced == [it for it in stored if it + 1 <= nt - 1]
+    quantised = [(r["it"] - 2) // store for r in fwd if r["qout"] >= 0]
+    assert quantised == [k for k, it in enumerate(stored) if it + 2 <= nt - 1]
+    # double buffers: a step never writes the parity it still has to read
+    for r in fwd:
+        if r["wmax"] >= 0 and r["red_in"] >= 0:
+            assert r["wmax"] != r["red_in"]
+        if r["red_out"] >= 0 and r["qparts"] >= 0:
+            assert r["red_out"] != r["qparts"]
+    # the bookkeeping tick settles the rest: slot nt-2's codes from its field,
+    # still in the buffer that forward step nt-2 wrote (not overwritten by
+    # step nt-1), and slot nt-1's slices for backward step 0
+    if nt >= 2 and (nt - 2) % store == 0:
+        assert any(v == ("F", nt - 2) for v in last_fwd.values())
+    # backward: each stored slot imaged once, the last forward field pending
+    imaged = {}
+    for s, r in enumerate(bwd):
+        it = r["it"]
+        if it in stored:
+            k = it // store
+            if s == 0:
+                assert r["pending"] >= 0 and r["s16in"] == -1 and r["qparts"] == (k & 1) and r["qcount"] == 2 + (k & 1)
+            else:
+                assert r["pending"] == -1 and r["s16in"] == k and r["s16max"] == k
+            imaged[k] = imaged.get(k, 0) + 1
+        else:
+            assert r["s16in"] == -1 and r["pending"] == -1
+    assert imaged == {k: 1 for k in range(nsnap)}
