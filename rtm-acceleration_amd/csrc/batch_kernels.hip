@@ -38,7 +38,12 @@ __global__ __launch_bounds__(64) void k_tick_args(const JobDesc* jobs, int njobs
              : (s16 && total == J.nt) ? kTickBookkeeping
                                       : kTickBackward;
     T.tile_first = J.tile_first;
-    T.a = s16 ? job_args<true>(J, total, dry) : job_args<false>(J, total, dry);
+    if (T.kind == kTickForward || T.kind == kTickBackward) {
+        T.a = s16 ? job_args<true>(J, total, dry) : job_args<false>(J, total, dry);
+    } else {                                 // bookkeeping reads the descriptor; idle nothing
+        T.a = StepArgs{};
+        T.a.dry = dry;
+    }
 }
 
 // Job clocks: every job has done `ticks` more steps.
