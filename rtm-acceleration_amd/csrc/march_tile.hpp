@@ -373,24 +373,27 @@ __device__ __forceinline__ float load_cur(const StepArgs& a, int x, int z, bool 
 // stores and loads are non-temporal, so they do not displace the fields,
 // tables and image that every step re-reads from the caches.
 //
-// Leapfrog update, damping, injection and the snapshot / imaging epilogue of
-// one output cell (x, z) (src/rtm/forward.cpp:45, :56-63, :112-118;
-// src/rtm/backward.cpp:57-75):
+// Leapfrog update and damping of one output cell (src/rtm/forward.cpp:45,
+// :56-63; src/rtm/backward.cpp:57-64):
 //   p^{n+1} = w ((2 p^n - w p^{n-1}) + (v dt)^2 lap), w = wx[x] wz[z]
 // (p^{n-1} is damped on read: the reference damps it in place after the
 // previous step, which gives the same value; injection cells lie where
 // w == 1, so injecting after the damping multiply is the reference's
-// inject-then-damp).  Entries [e0, e1) may hit this cell.
-template <Dir D, bool S16>
-__device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
-                                            float pc, float lap, const ColumnOps& o, float wzv,
-                                            const TileEntries& te, int e0, int e1, float& vmax) {
-    const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
+// inject-then-damp).
+__device__ __forceinline__ float leapfrog(float pc, float lap, const ColumnOps& o, float wzv) {
     const float w = o.wx * wzv;
     const float prev = o.pv * w;
-    float out = (2.0f * pc - prev) + o.vv * lap;
-    out = out * w;
-    out = apply_entries<D>(a, te, (int)i, out, o.vv, e0, e1);
+    const float out = (2.0f * pc - prev) + o.vv * lap;
+    return out * w;
+}
+
+// p^{n+1} of output cell (x, z), after injection, to the field, and the
+// snapshot / imaging epilogue of the cell (src/rtm/forward.cpp:112-118;
+// src/rtm/backward.cpp:66-75).
+template <Dir D, bool S16>
+__device__ __forceinline__ void store_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
+                                           float out, const ColumnOps& o, float& vmax) {
+    const ptrdiff_t i = (ptrdiff_t)x * a.ldz + z;
     gbl(a.p_next)[i] = out;
 
     const int jx = x - a.nb;
@@ -411,6 +414,69 @@ __device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc
                 gbl(a.illum)[j] = o.sv + fv * fv;
             }
         }
+    }
+}
+
+// One output cell with its injection entries [e0, e1), applied per cell.
+template <Dir D, bool S16>
+__device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
+                                            float pc, float lap, const ColumnOps& o, float wzv,
+                                            const TileEntries& te, int e0, int e1, float& vmax) {
+    float out = leapfrog(pc, lap, o, wzv);
+    out = apply_entries<D>(a, te, x * a.ldz + z, out, o.vv, e0, e1);
+    store_cell<D, S16>(a, sc, x, z, z_int, jz, out, o, vmax);
+}
+
+// Entry (en_cell, en_code, amp) on the TX output cells of a lane (cell[r]:
+// the cell index of column r, -1 where the lane outputs none): an entry hits
+// at most one of them; every compare and add is a select, no branch.
+template <Dir D, int TX>
+__device__ __forceinline__ void entry_on_cells(const StepArgs& a, const int (&cell)[TX], float (&out)[TX],
+                                               const ColumnOps (&o)[TX], int en_cell, int en_code, float amp) {
+    if (D == Dir::Forward && en_code >= 0) {
+        // trace recording: the cell's value at this point of its entry order
+        bool any = false;
+        float v = 0.0f;
+#pragma unroll
+        for (int r = 0; r < TX; ++r) {
+            const bool hit = cell[r] == en_cell;
+            v = hit ? out[r] : v;
+            any = any || hit;
+        }
+        if (any) gbl(a.rec_out)[(ptrdiff_t)en_code * a.rec_stride + a.it] = v;
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < TX; ++r) {
+        const float inj = out[r] + o[r].vv * amp;
+        out[r] = cell[r] == en_cell ? inj : out[r];
+    }
+}
+
+// Source / receiver injection and trace recording of a whole tile, in one
+// pass over its entries [e0, e1) (sorted by cell, then code), so every cell
+// sees its own entries in entry order, as the per-cell loop of
+// apply_entries does; a per-cell loop costs each tile that holds entries
+// TX loops with their branches, and those tiles set the end of the launch.
+template <Dir D, int TX>
+__device__ __forceinline__ void apply_tile_entries(const StepArgs& a, const TileEntries& te, const int (&cell)[TX],
+                                                   float (&out)[TX], const ColumnOps (&o)[TX], int e0, int e1) {
+    const int e64 = min(e1, te.base + kTileZ);
+    for (int e = e0; e < e64; ++e) {
+        const int k = e - te.base;
+        const int en_cell = __builtin_amdgcn_readlane(te.cell, k);
+        const int en_code = D == Dir::Forward ? __builtin_amdgcn_readlane(te.code, k) : 0;
+        const float amp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(te.amp), k));
+        entry_on_cells<D, TX>(a, cell, out, o, en_cell, en_code, amp);
+    }
+    // entries past the wave's 64 lanes: scalar loads (e is uniform)
+    for (int e = e64; e < e1; ++e) {
+        const int en_cell = sld(&a.entries[e].cell);
+        const int en_code = sld(&a.entries[e].code);
+        float amp;
+        if constexpr (D == Dir::Backward) amp = sld(a.amps + (ptrdiff_t)a.it * a.amp_count + e);
+        else amp = en_code < 0 ? sld(a.wavelet + a.it) : 0.0f;
+        entry_on_cells<D, TX>(a, cell, out, o, en_cell, en_code, amp);
     }
 }
 
@@ -505,12 +571,29 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     entries_ready(te);
     float vmax = 0.0f;
 
+    if (e0 == e1) {
+        // no injection entries (nearly every tile): each cell is stored as
+        // soon as it is computed, which keeps the stores streaming
 #pragma unroll
-    for (int r = 0; r < TX; ++r) {
-        const float lap = laplacian<H, DPP>(cf, q, r + H);
-        const int x = x0 + r;
-        if (!z_out || x >= a.nxe - H) continue;
-        finish_cell<D, S16>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, e0, e1, vmax);
+        for (int r = 0; r < TX; ++r) {
+            const float lap = laplacian<H, DPP>(cf, q, r + H);
+            const int x = x0 + r;
+            if (!z_out || x >= a.nxe - H) continue;
+            store_cell<D, S16>(a, sc, x, z, z_int, jz, leapfrog(q[r + H], lap, ops[r], wzv), ops[r], vmax);
+        }
+    } else {
+        float out[TX];
+        int cell[TX];
+#pragma unroll
+        for (int r = 0; r < TX; ++r) {
+            const int x = x0 + r;
+            out[r] = leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv);
+            cell[r] = (z_out && x < a.nxe - H) ? x * a.ldz + z : -1;
+        }
+        apply_tile_entries<D, TX>(a, te, cell, out, ops, e0, e1);
+#pragma unroll
+        for (int r = 0; r < TX; ++r)
+            if (cell[r] >= 0) store_cell<D, S16>(a, sc, x0 + r, z, z_int, jz, out[r], ops[r], vmax);
     }
     finish_tile<S16>(a, sc, vmax, lane, t, a.ntiles);
     settle_scales<S16>(a, sc, t, a.ntiles, lane);
