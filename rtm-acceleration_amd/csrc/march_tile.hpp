@@ -571,31 +571,15 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     entries_ready(te);
     float vmax = 0.0f;
 
-    // Tiles without injection entries (nearly all) store each cell as soon
-    // as it is computed, which keeps the stores streaming.  A tile with
-    // entries applies them in branch-free passes: per cell for the FP32
-    // forward step, once over the tile otherwise (the faster of the two
-    // layouts for each kernel, measured on the 2301 x 751 shot; the layout
-    // of the rare entry tiles changes the code of the whole kernel).
     if (e0 == e1) {
+        // no injection entries (nearly every tile): each cell is stored as
+        // soon as it is computed, which keeps the stores streaming
 #pragma unroll
         for (int r = 0; r < TX; ++r) {
             const float lap = laplacian<H, DPP>(cf, q, r + H);
             const int x = x0 + r;
             if (!z_out || x >= a.nxe - H) continue;
             store_cell<D, S16>(a, sc, x, z, z_int, jz, leapfrog(q[r + H], lap, ops[r], wzv), ops[r], vmax);
-        }
-    } else if constexpr (D == Dir::Forward && !S16) {
-#pragma unroll
-        for (int r = 0; r < TX; ++r) {
-            const float lap = laplacian<H, DPP>(cf, q, r + H);
-            const int x = x0 + r;
-            if (!z_out || x >= a.nxe - H) continue;
-            float out[1] = {leapfrog(q[r + H], lap, ops[r], wzv)};
-            const int cell[1] = {x * a.ldz + z};
-            const ColumnOps o1[1] = {ops[r]};
-            apply_tile_entries<D, 1>(a, te, cell, out, o1, e0, e1);
-            store_cell<D, S16>(a, sc, x, z, z_int, jz, out[0], ops[r], vmax);
         }
     } else {
         float out[TX];
