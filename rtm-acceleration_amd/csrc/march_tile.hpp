@@ -538,7 +538,10 @@ __device__ __forceinline__ void settle_scales(const StepArgs& a, const SnapCtx& 
 // March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
-template <int H, int TX, Dir D, bool DPP, bool S16>
+// LEAN: a step with no snapshot work (a backward step that does not image,
+// an FP32 forward step that stores no snapshot), built without that
+// epilogue: a smaller kernel for three steps in four.
+template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
 __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
@@ -551,6 +554,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     const int jz = z - a.nb;                 // interior depth index
     const bool z_int = z_out && jz >= 0 && jz < a.nz;
     SnapCtx sc = snap_ctx<S16>(a, D);
+    if constexpr (LEAN) sc.imaging = sc.store32 = false;
     snap_prefetch<S16>(a, sc, lane);
     // The entry bounds are scalar loads: issued before the tile's loads they
     // are in flight with them (FP32 steps 1-2 % faster); the 16-bit build

@@ -134,14 +134,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 }
 
 // March mapping: one wave per tile (march_tile.hpp).
-template <int H, int TX, Dir D, bool DPP, bool S16>
+template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     const int wave = (int)threadIdx.x / kTileZ;
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (t >= a.ntiles) return;               // whole wave leaves together
-    march_tile<H, TX, D, DPP, S16>(a, a.coef, t, lane);
+    march_tile<H, TX, D, DPP, S16, LEAN>(a, a.coef, t, lane);
+}
+
+// Whether a step has no snapshot work, so the lean build can run it
+// (snap_ctx's tests, on the host): a backward step that does not image, an
+// FP32 forward step that stores no snapshot.
+inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
+    if (d == Dir::Forward) return !s16 && a.snap_out == nullptr;
+    return s16 ? (a.snap16_in == nullptr && a.pending_field == nullptr) : a.snap_in == nullptr;
 }
 
 // Stream mapping: one wave per strip of XS columns (march_tile.hpp).
@@ -188,6 +196,20 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
             case 8:  RTM_LAUNCH((k_march<H, 8, D, false, S16>)); break;
             case 16: RTM_LAUNCH((k_march<H, 16, D, false, S16>)); break;
             default: return hipErrorInvalidValue;
+        }
+    } else if (step_is_lean(D, a, S16)) {
+        // the DPP march family has lean builds for steps without snapshot work
+        if constexpr (D == Dir::Backward || !S16) {
+            switch (g.tile_x) {
+                case 2:  RTM_LAUNCH((k_march<H, 2, D, true, S16, true>)); break;
+                case 3:  RTM_LAUNCH((k_march<H, 3, D, true, S16, true>)); break;
+                case 4:  RTM_LAUNCH((k_march<H, 4, D, true, S16, true>)); break;
+                case 5:  RTM_LAUNCH((k_march<H, 5, D, true, S16, true>)); break;
+                case 6:  RTM_LAUNCH((k_march<H, 6, D, true, S16, true>)); break;
+                case 8:  RTM_LAUNCH((k_march<H, 8, D, true, S16, true>)); break;
+                case 16: RTM_LAUNCH((k_march<H, 16, D, true, S16, true>)); break;
+                default: return hipErrorInvalidValue;
+            }
         }
     } else {
         switch (g.tile_x) {
