@@ -382,8 +382,11 @@ void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepAr
 }
 
 // Runs one propagation (forward or backward) of h->nt steps.
+// snapshots false: a forward pass that stores no snapshots (modelling, as
+// the reference's model_shot passes no snapshot store, src/rtm/rtm.cpp:109):
+// the step arguments keep their null snapshot operands and FP32 format.
 int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
-             const rtm_hip::StepGeometry& g) {
+             const rtm_hip::StepGeometry& g, bool snapshots = true) {
     const int k = dir == Dir::Forward ? 0 : 1;
     rtm_hip::StepArgs a{};
     a.vdt2 = h->d_vdt2;
@@ -408,7 +411,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         hipError_t e = hipMemsetAsync(p, 0, h->n_fld * sizeof(float), h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "wavefield reset");
     }
-    if (dir == Dir::Forward && h->d_snap_max) {
+    if (dir == Dir::Forward && snapshots && h->d_snap_max) {
         hipError_t e = hipMemsetAsync(h->d_snap_max, 0, (size_t)h->nsnap * sizeof(float), h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "snapshot scale reset");
     }
@@ -431,7 +434,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         a.p_prev = h->d_fld[ia];
         a.p_cur = h->d_fld[ib];
         a.p_next = h->d_fld[ic];
-        set_snapshot_args(h, dir, it, a);
+        if (snapshots) set_snapshot_args(h, dir, it, a);
 
         const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
         EventPair ev;
@@ -454,7 +457,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     // their max from the slices (step nt-2) or the per-wave maxima (step
     // nt-1), then their codes from the field (p_cur / p_prev after the loop's
     // last rotation hold the fields of steps nt-1 / nt-2)
-    if (dir == Dir::Forward && h->snap_fmt != RTM_HIP_SNAP_F32) {
+    if (dir == Dir::Forward && snapshots && h->snap_fmt != RTM_HIP_SNAP_F32) {
         const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
         for (int it = std::max(0, h->nt - 2); it < h->nt; ++it) {
             if (it % h->store) continue;
@@ -598,7 +601,8 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     return RTM_HIP_OK;
 }
 
-int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out, float* recorded) {
+int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out, float* recorded,
+                 bool snapshots = true) {
     if (int rc = validate_shot(h, s, false)) return rc;
     rtm::Timer wall;
     if (int rc = autotune(h, Dir::Forward)) return rc;
@@ -611,7 +615,7 @@ int forward_impl(rtm_hip_engine* h, const rtm_hip_shot* s, float* snapshots_out,
         hipError_t e = hipMemsetAsync(h->d_rec, 0, n * sizeof(float), h->stream);
         if (e != hipSuccess) return h->hip_fail(e, "recorded traces reset");
     }
-    if (int rc = run_loop(h, Dir::Forward, s, record, g)) return rc;
+    if (int rc = run_loop(h, Dir::Forward, s, record, g, snapshots)) return rc;
     h->times.forward += wall.elapsed();
 
     rtm::Timer d2h;
@@ -972,7 +976,7 @@ int rtm_hip_model_shot(rtm_hip_engine* h, const rtm_hip_shot* shot, float* trace
     if (!traces_out) return h->fail(RTM_HIP_ERR_ARG, "null trace output");
     (void)hipSetDevice(h->device);
     if (shot && shot->nrec == 0) return RTM_HIP_OK;
-    return forward_impl(h, shot, nullptr, traces_out);
+    return forward_impl(h, shot, nullptr, traces_out, false);
 }
 
 int rtm_hip_get_times(const rtm_hip_engine* h, rtm_hip_times* out) {

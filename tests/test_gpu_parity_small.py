@@ -109,3 +109,24 @@ def test_source_on_receiver_cell_bit_identical(hip_engine_cls, family, tile):
         I, S = eng.migrate([shot])
     I_ref, S_ref = orc.migrate([shot])
     assert bits_equal(I_ref, I) and bits_equal(S_ref, S)
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_model_shot_leaves_snapshot_store(hip_engine_cls, fmt):
+    """Modelling stores no snapshots (the reference's model_shot passes no
+    snapshot store, src/rtm/rtm.cpp:105-111): a backward pass after
+    forward(A), model_shot(B) images A exactly as migrate([A])."""
+    print(f"case: model_shot between forward and backward, snapshot format {fmt}", flush=True)
+    model, params, shots = _case(8, 2)
+    with hip_engine_cls() as eng:
+        eng.setup(model, params, shots[0].nt, shots[0].dt, snapshot_format=fmt)
+        for s in shots:
+            s.traces = eng.model_shot(s)
+        I_ref, S_ref = eng.migrate([shots[0]])
+        eng.image_zero()
+        eng.forward(shots[0])
+        tr = eng.model_shot(shots[1])
+        eng.backward(shots[0])
+        I, S = eng.image()
+    assert bits_equal(tr, shots[1].traces)
+    assert bits_equal(I_ref, I) and bits_equal(S_ref, S)
