@@ -420,7 +420,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
                      "copy_probe_gbs": round(copy_peak, 1),
                      "frac_of_copy_probe": round(achieved / copy_peak, 4) if copy_peak else None,
-                     "kernel": "backward fused step (stencil + sponge + injection + imaging), order 8",
+                     "kernel": "backward fused step (stencil + sponge + injection + imaging), order 8; "
+                               "mean over the phase's launches: imaging build on stored steps, lean build "
+                               "on the others",
                      "bytes_per_launch": round(bwd_b), "mean_launch_us": round(bwd_ms * 1e3, 3),
                      "forward_kernel_us": round(fwd_ms * 1e3, 3),
                      "sampled_launches": ks["sampled"]},
