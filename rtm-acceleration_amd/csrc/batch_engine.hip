@@ -148,9 +148,15 @@ struct rtm_hip_batch {
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by group_key (half-width, snapshot width)
     bool dirty = true;
-    // captured ticks: graph[0] = one tick, graph[1] = kGraphTicks ticks
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    // captured runs of ticks, by the run's launch pattern: its length and,
+    // per tick and group, the lean or full build (a few patterns recur while
+    // the job set stays the same)
+    struct Captured {
+        std::string pattern;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Captured> graphs;
     int stack_nx = 0, stack_nz = 0;
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
@@ -185,12 +191,11 @@ struct rtm_hip_batch {
                     hipGetErrorString(e));
     }
     void drop_graph() {
-        for (int k = 0; k < 2; ++k) {
-            if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
-            if (graph[k]) (void)hipGraphDestroy(graph[k]);
-            graph_exec[k] = nullptr;
-            graph[k] = nullptr;
+        for (Captured& c : graphs) {
+            if (c.exec) (void)hipGraphExecDestroy(c.exec);
+            if (c.graph) (void)hipGraphDestroy(c.graph);
         }
+        graphs.clear();
     }
 };
 
@@ -550,9 +555,41 @@ int rebuild_tables(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
-// A run of `ticks` ticks (at most kRunTicks): the tick arguments of the run,
-// the group launches of every tick, then one clock update for the run.
-int enqueue_ticks(rtm_hip_batch* b, int ticks) {
+// Whether no job of group `key` has snapshot work `ahead` ticks after the
+// host's clocks (step_is_lean on the job's step arguments, as job_args sets
+// them up on the device), so the tick can run the group's lean build: no
+// FP32 job stores or images, no 16-bit job is in its forward phase or
+// images.  Jobs done by then idle, and a 16-bit job's bookkeeping tick does
+// not run the march tile.  Other kernel families have no lean build.
+bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead) {
+    if (g.family != rtm_hip::Family::kMarchDpp) return false;
+    const bool s16 = key_snap16(key);
+    for (const Job* j : b->active) {
+        if (group_key(*j->model) != key) continue;
+        const int total = j->desc.steps - j->steps_left + ahead;
+        if (total >= j->desc.steps || (s16 && total == j->desc.nt)) continue;
+        const rtm_hip::Dir d = total < j->desc.nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward;
+        const rtm_hip::StepArgs a = s16 ? rtm_hip::job_args<true>(j->desc, total, 0)
+                                        : rtm_hip::job_args<false>(j->desc, total, 0);
+        if (!rtm_hip::step_is_lean(d, a, s16)) return false;
+    }
+    return true;
+}
+
+// The launch pattern of a run of `ticks` ticks starting `ahead` ticks after
+// the host's clocks: its length, then per tick and group '1' (lean) or '0'.
+std::string run_pattern(const rtm_hip_batch* b, int ticks, int ahead) {
+    std::string p = std::to_string(ticks) + ":";
+    for (int sub = 0; sub < ticks; ++sub)
+        for (const auto& [key, g] : b->groups)
+            if (g.ntiles) p += group_tick_lean(b, key, g, ahead + sub) ? '1' : '0';
+    return p;
+}
+
+// A run of `ticks` ticks (at most kRunTicks) starting `ahead` ticks after the
+// host's clocks: the tick arguments of the run, the group launches of every
+// tick, then one clock update for the run.
+int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
     if (ticks < 1 || ticks > rtm_hip::kRunTicks)
         return b->fail(RTM_HIP_ERR_ARG, "internal: a run of %d ticks (1..%d)", ticks, rtm_hip::kRunTicks);
     hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, ticks, 0, b->stream);
@@ -560,9 +597,10 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks) {
     for (int sub = 0; sub < ticks; ++sub) {
         for (auto& [key, g] : b->groups) {
             if (g.ntiles == 0) continue;
+            const bool lean = group_tick_lean(b, key, g, ahead + sub);
             hipError_t e = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key), b->d_jobs,
                                                       b->d_targs, g.d_tile_job, g.ntiles, sub, g.lds_bytes,
-                                                      b->stream);
+                                                      b->stream, lean);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
         }
     }
@@ -757,20 +795,42 @@ int retire_finished(rtm_hip_batch* b) {
     return RTM_HIP_OK;
 }
 
-// Captures `ticks` consecutive ticks into graph slot k.
-int capture_ticks_now(rtm_hip_batch* b, int k, int ticks);
+// The captured graph of a run of `ticks` ticks starting `ahead` ticks after
+// the host's clocks: a graph of the same launch pattern if one was captured
+// since the job set last changed, else a new capture (the tick arguments are
+// computed on the device from the job clocks, so a graph serves every run of
+// its pattern).  At most kMaxGraphs patterns are kept.
+constexpr size_t kMaxGraphs = 8;
+int capture_ticks_now(rtm_hip_batch* b, int ticks, int ahead, rtm_hip_batch::Captured& c);
 
-int capture_ticks(rtm_hip_batch* b, int k, int ticks) {
+int graph_for(rtm_hip_batch* b, int ticks, int ahead, hipGraphExec_t* exec) {
+    std::string pattern = run_pattern(b, ticks, ahead);
+    for (const auto& c : b->graphs)
+        if (c.pattern == pattern) {
+            *exec = c.exec;
+            return RTM_HIP_OK;
+        }
+    if (b->graphs.size() >= kMaxGraphs) {
+        // the kept graphs may still run on the stream
+        if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess)
+            return b->hip_fail(e, "batch synchronisation");
+        b->drop_graph();
+    }
+    rtm_hip_batch::Captured c;
+    c.pattern = std::move(pattern);
     const double t0 = now_s();
-    const int rc = capture_ticks_now(b, k, ticks);
+    const int rc = capture_ticks_now(b, ticks, ahead, c);
     b->stats.capture_seconds += now_s() - t0;
-    return rc;
+    if (rc) return rc;
+    b->graphs.push_back(c);
+    *exec = c.exec;
+    return RTM_HIP_OK;
 }
 
-int capture_ticks_now(rtm_hip_batch* b, int k, int ticks) {
+int capture_ticks_now(rtm_hip_batch* b, int ticks, int ahead, rtm_hip_batch::Captured& c) {
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-    const int rc = enqueue_ticks(b, ticks);
+    const int rc = enqueue_ticks(b, ticks, ahead);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -778,10 +838,10 @@ int capture_ticks_now(rtm_hip_batch* b, int k, int ticks) {
         return rc;
     }
     if (e != hipSuccess) return b->hip_fail(e, "graph capture");
-    b->graph[k] = g;
-    e = hipGraphInstantiate(&b->graph_exec[k], g, nullptr, nullptr, 0);
+    c.graph = g;
+    e = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        b->drop_graph();
+        (void)hipGraphDestroy(g);
         return b->hip_fail(e, "graph instantiation");
     }
     return RTM_HIP_OK;
@@ -835,24 +895,17 @@ int run_ticks(rtm_hip_batch* b, int n) {
     if (b->active.size() == 1) {
         if (int rc = run_solo(b, n)) return rc;
     } else if (b->cfg.use_graph) {
-        int left = n;
-        if (left >= kGraphTicks) {
-            if (!b->graph_exec[1])
-                if (int rc = capture_ticks(b, 1, kGraphTicks)) return rc;
-            for (; left >= kGraphTicks; left -= kGraphTicks) {
-                hipError_t e = hipGraphLaunch(b->graph_exec[1], b->stream);
-                if (e != hipSuccess) return b->hip_fail(e, "graph launch");
-            }
-        }
-        if (left > 0 && !b->graph_exec[0])
-            if (int rc = capture_ticks(b, 0, 1)) return rc;
-        for (; left > 0; --left) {
-            hipError_t e = hipGraphLaunch(b->graph_exec[0], b->stream);
+        for (int ahead = 0; ahead < n;) {
+            const int ticks = n - ahead >= kGraphTicks ? kGraphTicks : 1;
+            hipGraphExec_t exec = nullptr;
+            if (int rc = graph_for(b, ticks, ahead, &exec)) return rc;
+            hipError_t e = hipGraphLaunch(exec, b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "graph launch");
+            ahead += ticks;
         }
     } else {
-        for (int left = n; left > 0; left -= kGraphTicks)
-            if (int rc = enqueue_ticks(b, std::min(left, kGraphTicks))) return rc;
+        for (int ahead = 0; ahead < n; ahead += kGraphTicks)
+            if (int rc = enqueue_ticks(b, std::min(n - ahead, kGraphTicks), ahead)) return rc;
     }
     for (Job* j : b->active) j->steps_left -= n;
     b->stats.ticks += n;

@@ -66,9 +66,11 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* image, const flo
 
 hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, const JobDesc* jobs,
                              const TickArgs* targs, const int* tile_job, int ntiles, int sub, int lds_bytes,
-                             hipStream_t stream) {
-    return snap16 ? launch_batch_step_s16(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream)
-                  : launch_batch_step_f32(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream);
+                             hipStream_t stream, bool lean) {
+    return snap16 ? launch_batch_step_s16(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream,
+                                          lean)
+                  : launch_batch_step_f32(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream,
+                                          lean);
 }
 
 hipError_t launch_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int nsub, int dry,

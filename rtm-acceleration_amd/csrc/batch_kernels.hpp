@@ -84,13 +84,17 @@ hipError_t launch_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int
 // dynamic LDS per workgroup cap the workgroups resident on a CU.
 // snap16 selects the build for jobs with 16-bit snapshots (every job of
 // the launch must have the same snapshot width).
+// lean: no job of the group has snapshot work on this tick (groups of the
+// DPP march family then run a build without the snapshot epilogue).
 hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, const JobDesc* jobs,
                              const TickArgs* targs, const int* tile_job, int ntiles, int sub, int lds_bytes,
-                             hipStream_t stream);
+                             hipStream_t stream, bool lean = false);
 hipError_t launch_batch_step_f32(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
-                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream);
+                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream,
+                                 bool lean);
 hipError_t launch_batch_step_s16(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
-                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream);
+                                 const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream,
+                                 bool lean);
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]
 // (and the same for illumination), for the job's nx x nz image.

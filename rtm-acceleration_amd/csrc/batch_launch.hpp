@@ -53,8 +53,9 @@ __device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int la
 // Tick `sub` of a run of ticks for every job of one group: the wave's job
 // (wave-uniform, read-first-lane) and that job's tick arguments, read with
 // independent scalar loads; then the same march/stream tile as the per-shot
-// kernel.
-template <int H, int TX, Family F, bool S16>
+// kernel.  LEAN: a tick in which no job of the group has snapshot work, run
+// on the lean build of the march tile.
+template <int H, int TX, Family F, bool S16, bool LEAN = false>
 __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restrict__ jobs,
                                                          const TickArgs* __restrict__ targs,
                                                          const int* __restrict__ tile_job, int ntiles, int sub) {
@@ -78,18 +79,34 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restri
         else stream_tile<H, TX, Dir::Backward, S16>(a, a.coef, t, lane);
     } else {
         constexpr bool DPP = F == Family::kMarchDpp;
-        if (kind == kTickForward) march_tile<H, TX, Dir::Forward, DPP, S16>(a, a.coef, t, lane);
-        else march_tile<H, TX, Dir::Backward, DPP, S16>(a, a.coef, t, lane);
+        if (kind == kTickForward) march_tile<H, TX, Dir::Forward, DPP, S16, LEAN>(a, a.coef, t, lane);
+        else march_tile<H, TX, Dir::Backward, DPP, S16, LEAN>(a, a.coef, t, lane);
     }
 }
 
 template <int H, bool S16>
 hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs, const int* tile_job,
-                    int ntiles, int sub, size_t lds, hipStream_t s) {
+                    int ntiles, int sub, size_t lds, hipStream_t s, bool lean) {
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
 #define RTM_BATCH(TX, F) \
     hipLaunchKernelGGL((k_batch<H, TX, F, S16>), grid, block, lds, s, jobs, targs, tile_job, ntiles, sub)
+#define RTM_BATCH_LEAN(TX) \
+    hipLaunchKernelGGL((k_batch<H, TX, Family::kMarchDpp, S16, true>), grid, block, lds, s, jobs, targs, tile_job, \
+                       ntiles, sub)
+    if (lean && family == Family::kMarchDpp) {          // the DPP march family has lean builds
+        switch (tile_x) {
+            case 2: RTM_BATCH_LEAN(2); break;
+            case 3: RTM_BATCH_LEAN(3); break;
+            case 4: RTM_BATCH_LEAN(4); break;
+            case 5: RTM_BATCH_LEAN(5); break;
+            case 6: RTM_BATCH_LEAN(6); break;
+            case 8: RTM_BATCH_LEAN(8); break;
+            case 16: RTM_BATCH_LEAN(16); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (family == Family::kStream) {
         switch (tile_x) {
             case 8: RTM_BATCH(8, Family::kStream); break;
@@ -119,22 +136,24 @@ hipError_t launch_h(Family family, int tile_x, const JobDesc* jobs, const TickAr
         return hipErrorInvalidValue;
     }
 #undef RTM_BATCH
+#undef RTM_BATCH_LEAN
     return hipGetLastError();
 }
 
 
 template <bool S16>
 hipError_t launch_batch_step_t(int half, Family family, int tile_x, const JobDesc* jobs, const TickArgs* targs,
-                               const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream) {
+                               const int* tile_job, int ntiles, int sub, int lds_bytes, hipStream_t stream,
+                               bool lean) {
     const size_t lds = (size_t)lds_bytes;
     if (ntiles <= 0) return hipSuccess;
     switch (half) {
-        case 1: return launch_h<1, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
-        case 2: return launch_h<2, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
-        case 3: return launch_h<3, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
-        case 4: return launch_h<4, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
-        case 5: return launch_h<5, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
-        case 6: return launch_h<6, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream);
+        case 1: return launch_h<1, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
+        case 2: return launch_h<2, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
+        case 3: return launch_h<3, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
+        case 4: return launch_h<4, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
+        case 5: return launch_h<5, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
+        case 6: return launch_h<6, S16>(family, tile_x, jobs, targs, tile_job, ntiles, sub, lds, stream, lean);
         default: return hipErrorInvalidValue;
     }
 }

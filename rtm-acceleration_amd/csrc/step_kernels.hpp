@@ -112,6 +112,15 @@ struct StepArgs {
     int          dry;
 };
 
+// Whether a step has no snapshot work, so a lean build of the step (one
+// without the snapshot epilogue) can run it: a backward step that does not
+// image, an FP32 forward step that stores no snapshot.  The same tests as
+// the kernels' own (snap_ctx), on the host.
+inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
+    if (d == Dir::Forward) return !s16 && a.snap_out == nullptr;
+    return s16 ? (a.snap16_in == nullptr && a.pending_field == nullptr) : a.snap_in == nullptr;
+}
+
 // Kernel families of the fused step.
 //   kLdsTile: a 256-thread workgroup stages a 64 x tile_x tile of p^n (+halo)
 //             in LDS; tiles start at cell (0, 0) and cover the padded grid.

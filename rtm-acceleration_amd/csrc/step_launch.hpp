@@ -144,13 +144,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     march_tile<H, TX, D, DPP, S16, LEAN>(a, a.coef, t, lane);
 }
 
-// Whether a step has no snapshot work, so the lean build can run it
-// (snap_ctx's tests, on the host): a backward step that does not image, an
-// FP32 forward step that stores no snapshot.
-inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
-    if (d == Dir::Forward) return !s16 && a.snap_out == nullptr;
-    return s16 ? (a.snap16_in == nullptr && a.pending_field == nullptr) : a.snap_in == nullptr;
-}
 
 // Stream mapping: one wave per strip of XS columns (march_tile.hpp).
 template <int H, int XS, Dir D, bool S16>

@@ -3,7 +3,8 @@
 // named on the command line as nt:store:fmt.  Buffers are fake addresses, so
 // each pointer prints as the index of what it points at; the test checks
 // the schedule's dataflow (which field each step reads and writes, which
-// snapshot slot and scale buffers) on the host.
+// snapshot slot and scale buffers) on the host, and which steps may run a
+// lean build (step_is_lean).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -55,7 +56,7 @@ int main(int argc, char** argv) {
                 "\"zp\":%d,\"zc\":%d,\"snap_out\":%lld,\"snap_in\":%lld,\"wmax\":%lld,\"wcount\":%lld,"
                 "\"red_in\":%lld,\"red_count\":%lld,\"red_out\":%lld,\"red_out_count\":%lld,"
                 "\"qparts\":%lld,\"qcount\":%lld,\"qout\":%lld,\"qmax\":%lld,"
-                "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld}\n",
+                "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld,\"lean\":%d}\n",
                 argv[c], total, total < nt ? "fwd" : "bwd", a.it, fld(a.p_prev), fld(a.p_cur), fld(a.p_next),
                 a.zero_prev, a.zero_cur, idx(a.snap_out, 0x10000000u, kInt * 4), idx(a.snap_in, 0x10000000u, kInt * 4),
                 idx(a.wave_max, 0x30000000u, kCap * 4), idx(a.wave_count, 0x50000000u, 4),
@@ -63,7 +64,8 @@ int main(int argc, char** argv) {
                 idx(a.red_out, 0x40000000u, rtm_hip::kReduceWaves * 4), idx(a.red_out_count, 0x50000000u, 4),
                 idx(a.quant_parts, 0x40000000u, rtm_hip::kReduceWaves * 4), idx(a.quant_count, 0x50000000u, 4),
                 idx(a.quant_out, 0x20000000u, kInt * 2), idx(a.quant_max_out, 0x60000000u, 4),
-                idx(a.snap16_in, 0x20000000u, kInt * 2), idx(a.snap16_max, 0x60000000u, 4), fld(a.pending_field));
+                idx(a.snap16_in, 0x20000000u, kInt * 2), idx(a.snap16_max, 0x60000000u, 4), fld(a.pending_field),
+                (int)rtm_hip::step_is_lean(total < nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, a, s16));
         }
     }
     return 0;

@@ -9,6 +9,10 @@ restating its formulas:
 * FP32: every snapshot slot is written once in the forward phase by the
   step that produces that field and read once in the backward phase at the
   matching time;
+* a step may run a lean build (no snapshot epilogue; the batch engine picks
+  it per tick, the plain engine per launch) exactly when it has no snapshot
+  slot to write or image: FP32 steps that are not stored, 16-bit backward
+  steps that do not image, never a 16-bit forward step;
 * 16-bit: the scale of slot k is reduced from the per-wave maxima that step
   it_k wrote (same parity) and the codes are written from the field of step
   it_k, two steps later, with the slice maxima of the same parity; the last
@@ -75,6 +79,16 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
             content[r["next"]] = (phase, s)
         if phase == "F":
             last_fwd = dict(content)
+
+    for r in fwd + bwd:
+        if s16 and r["kind"] == "fwd":
+            assert r["lean"] == 0
+        elif s16:
+            assert r["lean"] == (r["s16in"] == -1 and r["pending"] == -1)
+        else:
+            assert r["lean"] == (r["snap_out"] == -1 and r["snap_in"] == -1)
+        if r["kind"] == "bwd" or not s16:
+            assert r["lean"] == (r["it"] not in stored)
 
     if not s16:
         assert [r["snap_out"] for r in fwd] == [it // store if it in stored else -1 for it in range(nt)]
