@@ -145,6 +145,7 @@ def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
             "snapshots": ["f32", "f16", "bf16"][snapshot_format], "groups": st["groups"],
+            "lean_launch_fraction": round(st["lean_launches"] / max(st["step_launches"], 1), 4),
             "image_checksum": float(np.abs(img).sum())}
 
 

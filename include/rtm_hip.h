@@ -278,6 +278,9 @@ typedef struct {
     int       tunes;             /* autotune searches (one per group and load class) */
     double    rebuild_seconds;   /* launch-table rebuilds (inside run_seconds)      */
     double    capture_seconds;   /* hipGraph capture and instantiation (inside run) */
+    long long graphs_captured;   /* captured runs of ticks, one per launch pattern  */
+    long long step_launches;     /* batched step launches (one per tick and group)  */
+    long long lean_launches;     /* of which ran the lean build (no snapshot work)  */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

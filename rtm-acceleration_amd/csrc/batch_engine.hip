@@ -822,6 +822,7 @@ int graph_for(rtm_hip_batch* b, int ticks, int ahead, hipGraphExec_t* exec) {
     const int rc = capture_ticks_now(b, ticks, ahead, c);
     b->stats.capture_seconds += now_s() - t0;
     if (rc) return rc;
+    ++b->stats.graphs_captured;
     b->graphs.push_back(c);
     *exec = c.exec;
     return RTM_HIP_OK;
@@ -891,6 +892,14 @@ int run_solo(rtm_hip_batch* b, int n) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
+// Step launches of a run of ticks, and how many ran the lean build.
+void count_launches(rtm_hip_batch* b, int ticks, int ahead) {
+    const std::string p = run_pattern(b, ticks, ahead);
+    const size_t body = p.find(':') + 1;
+    b->stats.step_launches += (long long)(p.size() - body);
+    b->stats.lean_launches += (long long)std::count(p.begin() + (ptrdiff_t)body, p.end(), '1');
+}
+
 int run_ticks(rtm_hip_batch* b, int n) {
     if (b->active.size() == 1) {
         if (int rc = run_solo(b, n)) return rc;
@@ -901,11 +910,15 @@ int run_ticks(rtm_hip_batch* b, int n) {
             if (int rc = graph_for(b, ticks, ahead, &exec)) return rc;
             hipError_t e = hipGraphLaunch(exec, b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "graph launch");
+            count_launches(b, ticks, ahead);
             ahead += ticks;
         }
     } else {
-        for (int ahead = 0; ahead < n; ahead += kGraphTicks)
-            if (int rc = enqueue_ticks(b, std::min(n - ahead, kGraphTicks), ahead)) return rc;
+        for (int ahead = 0; ahead < n; ahead += kGraphTicks) {
+            const int ticks = std::min(n - ahead, kGraphTicks);
+            if (int rc = enqueue_ticks(b, ticks, ahead)) return rc;
+            count_launches(b, ticks, ahead);
+        }
     }
     for (Job* j : b->active) j->steps_left -= n;
     b->stats.ticks += n;

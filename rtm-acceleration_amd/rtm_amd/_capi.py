@@ -77,7 +77,9 @@ class BatchStats(C.Structure):
                 ("bytes_budget", C.c_size_t), ("ngroups", C.c_int), ("group_half", C.c_int * 12),
                 ("group_tile_x", C.c_int * 12), ("group_family", C.c_int * 12),
                 ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int),
-                ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double)]
+                ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double),
+                ("graphs_captured", C.c_longlong), ("step_launches", C.c_longlong),
+                ("lean_launches", C.c_longlong)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

@@ -331,3 +331,31 @@ def test_reserved_arena(jobs, alone):
                 I, S = b.result(jid)
                 assert bits_equal(alone[k][0], I), f"job {k}, reserve {reserve}"
                 assert bits_equal(alone[k][1], S), f"job {k}, reserve {reserve}"
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_lean_tick_patterns(hip_engine_cls, fmt, use_graph):
+    """Ticks in which no job of a group stores or images run the lean build.
+    Three jobs of one group with store intervals 3, 5 and 7 and different nt
+    make the lean/full pattern of almost every 32-tick run a new one, so the
+    per-pattern graph cache fills and is emptied mid-batch; every image equals
+    the job's migration alone."""
+    import rtm_amd
+    print(f"case: lean tick patterns, snapshot format {fmt}, graph={use_graph}", flush=True)
+    specs = [(48, 36, 8, 12, 300, 3), (52, 30, 8, 12, 331, 5), (44, 40, 8, 12, 362, 7)]
+    jobs = [_job(20 + k, nx, nz, order, nb, nt, store, fmt, 0) for k, (nx, nz, order, nb, nt, store) in
+            enumerate(specs)]
+    alone = [_alone(hip_engine_cls, *j) for j in jobs]
+    with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+        ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+        b.run()
+        st = b.stats()
+        assert st["jobs_done"] == len(jobs)
+        assert 0 < st["lean_launches"] < st["step_launches"]
+        if use_graph and fmt == 0:
+            assert st["graphs_captured"] > 8           # the pattern cache was emptied mid-batch
+        for k, jid in enumerate(ids):
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}: image max diff {np.abs(alone[k][0] - I).max()}"
+            assert bits_equal(alone[k][1], S), f"job {k}: illumination differs"
