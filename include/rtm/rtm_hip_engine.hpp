@@ -43,6 +43,12 @@ public:
                  std::vector<float>& illumination) override;
     void model_shot(ShotRecord& shot) override;
 
+    // Illumination compensation and/or the Laplacian filter
+    // (src/rtm/imaging.cpp:48-75) of the image left on the device by the last
+    // migrate, computed on the device; flags are RTM_HIP_POST_* (rtm_hip.h).
+    // out is assigned nx*nz values, bit-identical to the host functions.
+    void postprocess(int flags, float eps_relative, std::vector<float>& out);
+
     rtm_hip_engine* handle() const { return h_; }
 
 private:

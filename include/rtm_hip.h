@@ -164,6 +164,20 @@ int         rtm_hip_image_get(rtm_hip_engine* h, float* image, float* illum);
  * counterpart: the reference stacks in FP32). */
 int         rtm_hip_image_get_f64(rtm_hip_engine* h, double* image, double* illum);
 
+/* Post-processing of the resident image on the device.  Replaces
+ * illumination_compensation and laplacian_filter (src/rtm/imaging.cpp:48-75),
+ * which the reference's rtm applies on the host to copies of the stacked
+ * image for its _illum.bin and _filtered.bin outputs (src/main.cpp:211-222):
+ *   RTM_HIP_POST_ILLUM      out = I / (S + eps_relative*max(S) + 1e-30f)
+ *   RTM_HIP_POST_LAPLACIAN  out = Laplacian high-pass of I (of the compensated
+ *                           image when both flags are set), 0 on the outer ring
+ * out receives nx*nz floats; the device image and illumination are left
+ * unchanged.  With an FP64 stack the source is the stack rounded to float, as
+ * rtm_hip_image_get returns it.  Bit-identical to the reference's host code. */
+#define RTM_HIP_POST_ILLUM      1
+#define RTM_HIP_POST_LAPLACIAN  2
+int         rtm_hip_image_postprocess(rtm_hip_engine* h, int flags, float eps_relative, float* out);
+
 /* Replaces RTMEngine::migrate (include/rtm_engine.hpp:45-47; src/rtm/rtm.cpp:83-103;
  * src/cuda/rtm_cuda.cu:214-250): zero the image, forward + backward every shot
  * in order (FP32 running sum on the device), download image and illum. */

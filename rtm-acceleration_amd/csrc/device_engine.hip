@@ -25,6 +25,7 @@
 #include "rtm/benchmark.hpp"
 #include "rtm/rtm_types.hpp"
 #include "rtm_hip.h"
+#include "image_post.hpp"
 #include "step_kernels.hpp"
 
 using rtm_hip::Dir;
@@ -81,6 +82,8 @@ struct rtm_hip_engine {
     int image_stack = 0;             // 0: FP32 running sum, 1: FP64 stack
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
+    float* d_post = nullptr;         // post-processing scratch + output (allocated on first use)
+    size_t post_cap = 0;
     int* d_tile_ptr[2] = {nullptr, nullptr};
     size_t tile_ptr_cap[2] = {0, 0};
     Entry* d_ent[2] = {nullptr, nullptr};
@@ -165,7 +168,8 @@ struct rtm_hip_engine {
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
         f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_slice_max); f(d_part_counts); f(d_image); f(d_illum); f(d_traces); f(d_amps); f(d_rec);
-        f(d_stack_i); f(d_stack_s);
+        f(d_stack_i); f(d_stack_s); f(d_post);
+        post_cap = 0;
         for (int k = 0; k < 2; ++k) {
             f(d_tile_ptr[k]); f(d_ent[k]);
             tile_ptr_cap[k] = ent_cap[k] = 0;
@@ -950,6 +954,30 @@ int rtm_hip_image_get_f64(rtm_hip_engine* h, double* image, double* illum) {
         e = hipMemcpy(illum, h->d_stack_s, h->n_int * sizeof(double), hipMemcpyDeviceToHost);
     h->times.d2h += t.elapsed();
     return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image download");
+}
+
+int rtm_hip_image_postprocess(rtm_hip_engine* h, int flags, float eps_relative, float* out) {
+    if (int rc = check_handle(h, true)) return rc;
+    if (!out) return h->fail(RTM_HIP_ERR_ARG, "null post-processing output");
+    if (flags <= 0 || (flags & ~(RTM_HIP_POST_ILLUM | RTM_HIP_POST_LAPLACIAN)))
+        return h->fail(RTM_HIP_ERR_ARG, "post-processing flags %d: expected RTM_HIP_POST_ILLUM and/or "
+                                        "RTM_HIP_POST_LAPLACIAN", flags);
+    if (h->nx > 65535) return h->fail(RTM_HIP_ERR_ARG, "post-processing supports nx <= 65535 (nx = %d)", h->nx);
+    (void)hipSetDevice(h->device);
+    const size_t scratch = rtm_hip::post_scratch_floats(h->n_int);
+    if (int rc = h->grow(&h->d_post, &h->post_cap, scratch + h->n_int, "post-processing buffer")) return rc;
+    float* d_out = h->d_post + scratch;
+    // the reference forms the inverse squared spacings on the host in float
+    // (src/rtm/imaging.cpp:65-66)
+    const float idx2 = 1.0f / (h->dx * h->dx);
+    const float idz2 = 1.0f / (h->dz * h->dz);
+    hipError_t e = rtm_hip::launch_image_post(
+        h->image_stack ? nullptr : h->d_image, h->image_stack ? nullptr : h->d_illum,
+        h->image_stack ? h->d_stack_i : nullptr, h->image_stack ? h->d_stack_s : nullptr, h->nx, h->nz, idx2, idz2,
+        flags, eps_relative, h->d_post, d_out, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image post-processing");
 }
 
 int rtm_hip_migrate(rtm_hip_engine* h, int nshots, const rtm_hip_shot* shots, float* image, float* illum) {

@@ -124,4 +124,9 @@ void HipRTM::model_shot(ShotRecord& shot) {
     pull_times();
 }
 
+void HipRTM::postprocess(int flags, float eps_relative, std::vector<float>& out) {
+    out.assign(static_cast<std::size_t>(g_.nx) * g_.nz, 0.0f);
+    check(rtm_hip_image_postprocess(h_, flags, eps_relative, out.data()));
+}
+
 }  // namespace rtm

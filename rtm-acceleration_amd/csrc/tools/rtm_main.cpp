@@ -14,6 +14,8 @@
 #include "rtm/benchmark.hpp"
 #include "rtm/comparison.hpp"
 #include "rtm/rtm_factory.hpp"
+#include "rtm/rtm_hip_engine.hpp"
+#include "rtm_hip.h"
 #include "rtm/rtm_io.hpp"
 
 using namespace rtm;
@@ -177,16 +179,29 @@ int main(int argc, char** argv) try {
     const auto dot = out_path.find_last_of('.');
     const std::string stem = dot == std::string::npos ? out_path : out_path.substr(0, dot);
     if (want_illum_map) io::write_raw_floats(stem + "_illummap.bin", illum);
+    // post-processed copies (src/main.cpp:211-222): on the device, from the
+    // image the HIP engine keeps resident after migrate
+    auto* hip = dynamic_cast<HipRTM*>(engine.get());
     if (want_illum) {
-        std::vector<float> tmp = image;
-        illumination_compensation(tmp, illum, 1e-4f);
+        std::vector<float> tmp;
+        if (hip) {
+            hip->postprocess(RTM_HIP_POST_ILLUM, 1e-4f, tmp);
+        } else {
+            tmp = image;
+            illumination_compensation(tmp, illum, 1e-4f);
+        }
         io::write_raw_floats(stem + "_illum.bin", tmp);
         std::printf("wrote %s_illum.bin\n", stem.c_str());
     }
     if (filter == "laplacian") {
-        std::vector<float> tmp = image;
-        if (want_illum) illumination_compensation(tmp, illum, 1e-4f);
-        laplacian_filter(tmp, g);
+        std::vector<float> tmp;
+        if (hip) {
+            hip->postprocess(RTM_HIP_POST_LAPLACIAN | (want_illum ? RTM_HIP_POST_ILLUM : 0), 1e-4f, tmp);
+        } else {
+            tmp = image;
+            if (want_illum) illumination_compensation(tmp, illum, 1e-4f);
+            laplacian_filter(tmp, g);
+        }
         io::write_raw_floats(stem + "_filtered.bin", tmp);
         std::printf("wrote %s_filtered.bin\n", stem.c_str());
     }

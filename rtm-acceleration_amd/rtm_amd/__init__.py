@@ -137,6 +137,14 @@ class HipEngine:
         self._check(self.lib.rtm_hip_image_get(self.h, _capi.fptr(img), _capi.fptr(ill)))
         return img, ill
 
+    def postprocess(self, illum: bool = True, laplacian: bool = False, eps_relative: float = 1e-4):
+        """Illumination compensation and/or the Laplacian filter of the
+        resident image, on the device (src/rtm/imaging.cpp:48-75)."""
+        flags = (_capi.POST_ILLUM if illum else 0) | (_capi.POST_LAPLACIAN if laplacian else 0)
+        out = np.empty(max(self.n_int, 1), np.float32)
+        self._check(self.lib.rtm_hip_image_postprocess(self.h, flags, eps_relative, _capi.fptr(out)))
+        return out[:self.n_int]
+
     def image_f64(self):
         """FP64 stacks (engines set up with image_stack=STACK_F64)."""
         n = self.n_int

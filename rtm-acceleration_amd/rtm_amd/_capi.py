@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "librtm_hip.so")
 OK, ERR_ARG, ERR_DEVICE, ERR_MEMORY = 0, -1, -2, -3
 STEP_LDS_TILE, STEP_MARCH, STEP_MARCH_DPP, STEP_STREAM = 0, 1, 2, 3
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
+POST_ILLUM, POST_LAPLACIAN = 1, 2
 STACK_F32, STACK_F64 = 0, 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
@@ -97,6 +98,7 @@ SIGNATURES = {
     "rtm_hip_image_zero": (C.c_int, [_H]),
     "rtm_hip_image_get": (C.c_int, [_H, _f32p, _f32p]),
     "rtm_hip_image_get_f64": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "rtm_hip_image_postprocess": (C.c_int, [_H, C.c_int, C.c_float, _f32p]),
     "rtm_hip_migrate": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _f32p, _f32p]),
     "rtm_hip_model_shot": (C.c_int, [_H, C.POINTER(Shot), _f32p]),
     "rtm_hip_get_times": (C.c_int, [_H, C.POINTER(Times)]),

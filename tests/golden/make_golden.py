@@ -10,7 +10,9 @@ sources).  Nothing here runs on the GPU host; the fixtures it writes are data:
     velocity.bin/.hdr  the model (written by rtm_amd.models.layered_3)
     image.f32          reference raw cross-correlation image, 201*151 float32
     illummap.f32       reference illumination map
-    manifest.json      settings + sha256 of velocity, shots file and outputs
+    manifest.json      settings + sha256 of velocity, shots file and outputs,
+                       also of the reference's post-processed images (--illumination,
+                       --filter laplacian with and without it), which are not kept
   tiny_cases.npz       small grids (orders 2..12, store intervals, duplicate and
                        clamped receivers, truncated nt): modelled traces and
                        migrated images from the reference "cpu" engine
@@ -87,17 +89,29 @@ def make_layered3() -> None:
              "--shots-out", shots, "--nt", "1400", "--dt", "0.001", "--f0", "12", "--nshots", "4",
              "--rec-dx", "10"])
         image = os.path.join(tmp, "image.bin")
-        run([os.path.join(REF_BIN, "rtm"), "--engine", "cpu", "--velocity", vel, "--shots", shots,
-             "--output", image, "--order", "8", "--nb", "60", "--f0", "12", "--store-interval", "2",
-             "--mute-direct", "1500", "--write-illum-map", "--quiet"])
+        mig = [os.path.join(REF_BIN, "rtm"), "--engine", "cpu", "--velocity", vel, "--shots", shots,
+               "--order", "8", "--nb", "60", "--f0", "12", "--store-interval", "2",
+               "--mute-direct", "1500", "--quiet"]
+        # with --illumination --filter laplacian the reference also writes the
+        # compensated image and the compensated + filtered image
+        run(mig + ["--output", image, "--write-illum-map", "--illumination", "--filter", "laplacian"])
         shutil.copy(image, os.path.join(out, "image.f32"))
         shutil.copy(os.path.join(tmp, "image_illummap.bin"), os.path.join(out, "illummap.f32"))
+        # a second run for the Laplacian filter of the raw image alone
+        image2 = os.path.join(tmp, "raw.bin")
+        run(mig + ["--output", image2, "--filter", "laplacian"])
+        if sha256_file(image2) != sha256_file(image):
+            raise RuntimeError("reference rtm is not reproducible")
         manifest = {
             "what": "reference CLI pipeline (rtm_synth then rtm --engine cpu), as scripts/run_synthetic.sh",
             "settings": settings,
             "sha256": {"velocity.bin": sha256_file(vel), "shots.rtms": sha256_file(shots),
                        "image.f32": sha256_file(image),
-                       "illummap.f32": sha256_file(os.path.join(tmp, "image_illummap.bin"))},
+                       "illummap.f32": sha256_file(os.path.join(tmp, "image_illummap.bin")),
+                       # post-processed outputs (src/main.cpp:211-222)
+                       "illum.f32": sha256_file(os.path.join(tmp, "image_illum.bin")),
+                       "filtered_illum.f32": sha256_file(os.path.join(tmp, "image_filtered.bin")),
+                       "filtered.f32": sha256_file(os.path.join(tmp, "raw_filtered.bin"))},
             "shots_bytes": os.path.getsize(shots),
         }
     with open(os.path.join(out, "manifest.json"), "w") as f:
@@ -218,6 +232,9 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref is not built; run make -C oracle where the reference sources exist")
     if "--only-builtin" in sys.argv:
         make_builtin()
+        sys.exit(0)
+    if "--only-layered3" in sys.argv:
+        make_layered3()
         sys.exit(0)
     make_layered3()
     make_builtin()

@@ -284,6 +284,21 @@ def mute_direct_wave(shot: Shot, v_surface: float, t_pad: float, t_taper: float)
 _ref = None
 
 
+def illumination_compensation(image: np.ndarray, illum: np.ndarray, eps_relative: float = 1e-4) -> np.ndarray:
+    """I / (S + eps_relative*max S + 1e-30) on a copy (src/rtm/imaging.cpp:48-56)."""
+    out = np.array(image, np.float32, copy=True).reshape(-1)
+    s = np.ascontiguousarray(illum, np.float32).reshape(-1)
+    lib().orc_illumination_compensation(out, s, out.size, eps_relative)
+    return out
+
+
+def laplacian_filter(image: np.ndarray, nx: int, nz: int, dx: float, dz: float) -> np.ndarray:
+    """Laplacian high-pass of a copy, 0 on the outer ring (src/rtm/imaging.cpp:63-75)."""
+    out = np.array(image, np.float32, copy=True).reshape(-1)
+    lib().orc_laplacian_filter(out, nx, nz, dx, dz)
+    return out
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_SO)
 

@@ -3,7 +3,8 @@
 * tiny grids, every order: traces and images bit-identical;
 * the 4-shot 201x151 workload: GPU-modelled shots reproduce the reference's
   RTMS checksum, and the GPU migration (after the direct-wave mute) reproduces
-  the reference image and illumination map bit for bit;
+  the reference image and illumination map bit for bit, and the device
+  post-processing reproduces the reference's compensated and filtered images;
 * Marmousi: the 12 GPU-modelled shots reproduce the reference-held sha256 of
   the dataset (data/real/marmousi_dataset.sha256:1).
 """
@@ -57,9 +58,15 @@ def test_layered3_pipeline(hip_engine_cls):
         image, illum = eng.migrate(shots)
         # run twice: no state leaks between migrations (scripts/run_synthetic.sh:26-40)
         image2, illum2 = eng.migrate(shots)
+        # the reference's post-processed outputs (src/main.cpp:211-222), on the device
+        post = {"illum.f32": eng.postprocess(illum=True),
+                "filtered.f32": eng.postprocess(illum=False, laplacian=True),
+                "filtered_illum.f32": eng.postprocess(illum=True, laplacian=True)}
     assert bits_equal(image, ref_image)
     assert bits_equal(illum, ref_illum)
     assert bits_equal(image2, image) and bits_equal(illum2, illum)
+    for name, arr in post.items():
+        assert G.sha256(arr.tobytes()) == man["sha256"][name], name
 
 
 def test_marmousi_dataset_checksum(hip_engine_cls):

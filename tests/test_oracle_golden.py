@@ -76,3 +76,17 @@ def test_marmousi_first_shot_checksum():
     p = O.Params(nb=s["nb"], order=s["order"], store_interval=1, f0=s["f0"])
     tr = O.Oracle(m, p, s["nt"], s["dt"], variant="cpu-opt").model_shot(shots[0])
     assert G.sha256(tr.astype(np.float32).tobytes()) == man["per_shot_traces_sha256"][0]
+
+
+def test_layered3_postprocessing():
+    """The oracle's illumination compensation and Laplacian filter, applied to
+    the reference image and illumination map, reproduce the post-processed
+    images the reference CLI wrote (--illumination, --filter laplacian, both;
+    src/main.cpp:211-222)."""
+    man, _, ref_image, ref_illum = G.layered3()
+    comp = O.illumination_compensation(ref_image, ref_illum, 1e-4)
+    filt = O.laplacian_filter(ref_image, 201, 151, 10.0, 10.0)
+    both = O.laplacian_filter(comp, 201, 151, 10.0, 10.0)
+    assert G.sha256(comp.tobytes()) == man["sha256"]["illum.f32"]
+    assert G.sha256(filt.tobytes()) == man["sha256"]["filtered.f32"]
+    assert G.sha256(both.tobytes()) == man["sha256"]["filtered_illum.f32"]

@@ -163,3 +163,28 @@ def test_mute_closed_form():
             u = (it - i0) / ntap
             assert out[ir, it] == pytest.approx(0.5 * (1 - math.cos(math.pi * u)), abs=2e-6)
         assert np.all(out[ir, min(i0 + ntap, nt):] == 1.0)
+
+
+def test_laplacian_filter_closed_form():
+    """On I = a*ix^2 + b*iz^2 + c*ix*iz + linear terms the second differences
+    are exact: the filter gives 2a/dx^2 + 2b/dz^2 inside and 0 on the outer
+    ring; a transposed axis or a wrong spacing changes the value."""
+    nx, nz, dx, dz = 9, 7, 2.0, 4.0
+    ix, iz = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    img = (3.0 * ix**2 + 5.0 * iz**2 + 1.0 * ix * iz + 7.0 * ix - 2.0 * iz + 1.0).astype(np.float32)
+    out = O.laplacian_filter(img.reshape(-1), nx, nz, dx, dz).reshape(nx, nz)
+    want = 2 * 3.0 / dx**2 + 2 * 5.0 / dz**2
+    assert np.all(out[1:-1, 1:-1] == np.float32(want))
+    assert not out[0].any() and not out[-1].any() and not out[:, 0].any() and not out[:, -1].any()
+
+
+def test_illumination_compensation_closed_form():
+    """I / (S + eps*max S + 1e-30): with S = 0 everywhere the divisor is 1e-30;
+    with max S = 4 and eps 0.5 it is S + 2."""
+    img = np.array([1.0, -2.0, 3.0, 0.0], np.float32)
+    S = np.array([0.0, 4.0, 2.0, 1.0], np.float32)
+    out = O.illumination_compensation(img, S, 0.5)
+    assert np.array_equal(out, (img / (S + np.float32(2.0) + np.float32(1e-30))).astype(np.float32))
+    assert out[1] == np.float32(-2.0 / 6.0)
+    z = O.illumination_compensation(img, np.zeros(4, np.float32), 1e-4)
+    assert np.array_equal(z, img / np.float32(1e-30))
