@@ -1,5 +1,5 @@
 // rtm_io.hpp -- velocity and shot-gather I/O and the direct-wave mute (same
-// surface as the reference's include/rtm_io.hpp:9-37; SEG-Y is not provided).
+// surface as the reference's include/rtm_io.hpp:9-37).
 #pragma once
 #include <string>
 #include <vector>
@@ -27,7 +27,12 @@ void          write_velocity_raw(const std::string& path, const VelocityModel& m
 // RTMS v1 little-endian shot container.
 void                    write_shots_raw(const std::string& path, const std::vector<ShotRecord>& shots);
 std::vector<ShotRecord> read_shots_raw(const std::string& path);
-// Dispatch on extension: only RTMS is supported; .sgy/.segy throw.
+// SEG-Y rev 0/1 (constant trace length; sample formats 1 IBM float, 2 int32,
+// 3 int16, 5 IEEE float): traces grouped into shots by field record number,
+// source and receiver x from the trace headers with the coordinate scalar,
+// depths from the arguments (src/io/seismic_io.cpp:126-249).
+std::vector<ShotRecord> read_shots_segy(const std::string& path, float src_depth, float rec_depth);
+// Dispatch on extension: .sgy/.segy -> SEG-Y, anything else -> RTMS.
 std::vector<ShotRecord> read_shots(const std::string& path, float src_depth, float rec_depth);
 
 void               write_raw_floats(const std::string& path, const std::vector<float>& v);

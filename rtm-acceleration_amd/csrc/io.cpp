@@ -1,6 +1,6 @@
 // io.cpp -- velocity models, RTMS shot gathers, raw float arrays, and the
-// direct-wave mute (the reference's src/io/velocity_io.cpp and
-// src/io/seismic_io.cpp:18-124; SEG-Y input is not provided).
+// direct-wave mute, and SEG-Y shot input (the reference's
+// src/io/velocity_io.cpp and src/io/seismic_io.cpp).
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -191,13 +191,132 @@ std::vector<ShotRecord> read_shots_raw(const std::string& path) {
     return shots;
 }
 
-std::vector<ShotRecord> read_shots(const std::string& path, float, float) {
+// ---- SEG-Y rev 0/1 input (src/io/seismic_io.cpp:126-249) -------------------
+// Layout: 3200-byte text header, 400-byte binary header, then traces of a
+// 240-byte header and ns samples, everything big-endian.  Read: the sample
+// interval (µs), samples per trace and sample format from the binary header
+// (file bytes 3217, 3221, 3225, 1-based); per trace the field record number
+// (bytes 9-12), the coordinate scalar (71-72), source x (73-76) and group x
+// (81-84).  Consecutive traces with the same record number form one shot; with
+// record number 0 a change of source x starts a new shot.  Depths are not read
+// from the file: they are the caller's src_depth and rec_depth.
+namespace {
+
+int16_t load_be16(const unsigned char* p) { return static_cast<int16_t>((p[0] << 8) | p[1]); }
+uint32_t load_be32u(const unsigned char* p) {
+    return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | uint32_t(p[3]);
+}
+int32_t load_be32(const unsigned char* p) { return static_cast<int32_t>(load_be32u(p)); }
+
+// IBM System/360 single precision: sign, 7-bit base-16 exponent biased by 64,
+// 24-bit fraction F with value 0.F * 16^(e-64).  The value is exact in double
+// (|exponent| <= 2^258), so it is formed exactly and rounded once to float.
+float ibm32_to_float(uint32_t w) {
+    const uint32_t frac = w & 0x00ffffffu;
+    const int e16 = static_cast<int>((w >> 24) & 0x7fu) - 64;
+    const double mag = std::ldexp(static_cast<double>(frac), 4 * e16 - 24);
+    return static_cast<float>((w >> 31) ? -mag : mag);
+}
+
+// Coordinate scalar: > 0 multiplies, < 0 divides by its magnitude, 0 leaves as is.
+float scaled_coordinate(int32_t c, int16_t scalar) {
+    if (scalar > 0) return static_cast<float>(c) * static_cast<float>(scalar);
+    if (scalar < 0) return static_cast<float>(c) / static_cast<float>(-scalar);
+    return static_cast<float>(c);
+}
+
+int segy_sample_bytes(int format) {
+    switch (format) {
+        case 1: case 2: case 5: return 4;
+        case 3: return 2;
+        default: return 0;
+    }
+}
+
+void decode_segy_samples(int format, const unsigned char* p, int ns, float* out) {
+    switch (format) {
+        case 1:
+            for (int k = 0; k < ns; ++k) out[k] = ibm32_to_float(load_be32u(p + 4 * k));
+            break;
+        case 5:
+            for (int k = 0; k < ns; ++k) {
+                const uint32_t bits = load_be32u(p + 4 * k);
+                std::memcpy(out + k, &bits, 4);
+            }
+            break;
+        case 2:
+            for (int k = 0; k < ns; ++k) out[k] = static_cast<float>(load_be32(p + 4 * k));
+            break;
+        case 3:
+            for (int k = 0; k < ns; ++k) out[k] = static_cast<float>(load_be16(p + 2 * k));
+            break;
+    }
+}
+
+}  // namespace
+
+std::vector<ShotRecord> read_shots_segy(const std::string& path, float src_depth, float rec_depth) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) throw std::runtime_error("cannot open SEG-Y: " + path);
+    const long long file_bytes = static_cast<long long>(f.tellg());
+    if (file_bytes < 3600) throw std::runtime_error("SEG-Y too short: " + path);
+    unsigned char bin[400];
+    f.seekg(3200);
+    f.read(reinterpret_cast<char*>(bin), sizeof bin);
+    const int dt_us = load_be16(bin + 16);
+    const int ns = load_be16(bin + 20);
+    const int format = load_be16(bin + 24);
+    if (ns <= 0) throw std::runtime_error("SEG-Y: invalid samples per trace");
+    if (dt_us <= 0) throw std::runtime_error("SEG-Y: invalid sample interval");
+    const int bps = segy_sample_bytes(format);
+    if (bps == 0) {
+        std::ostringstream m;
+        m << "SEG-Y sample format " << format << " is not supported (use 1, 2, 3 or 5)";
+        throw std::runtime_error(m.str());
+    }
+    const long long trace_bytes = 240 + static_cast<long long>(ns) * bps;
+    const long long ntraces = (file_bytes - 3600) / trace_bytes;
+    if (ntraces <= 0) throw std::runtime_error("SEG-Y contains no traces");
+
+    const float dt = static_cast<float>(dt_us) * 1e-6f;
+    std::vector<ShotRecord> shots;
+    std::vector<unsigned char> tr(static_cast<std::size_t>(trace_bytes));
+    int32_t shot_record = 0;
+    float shot_x = 0.0f;
+    f.seekg(3600);
+    for (long long t = 0; t < ntraces; ++t) {
+        if (!f.read(reinterpret_cast<char*>(tr.data()), static_cast<std::streamsize>(trace_bytes))) break;
+        const int32_t record = load_be32(tr.data() + 8);
+        const int16_t scalar = load_be16(tr.data() + 70);
+        const float sx = scaled_coordinate(load_be32(tr.data() + 72), scalar);
+        const float gx = scaled_coordinate(load_be32(tr.data() + 80), scalar);
+        if (shots.empty() || record != shot_record || (record == 0 && sx != shot_x)) {
+            ShotRecord s;
+            s.sx = sx;
+            s.sz = src_depth;
+            s.nt = ns;
+            s.dt = dt;
+            shots.push_back(std::move(s));
+            shot_record = record;
+            shot_x = sx;
+        }
+        ShotRecord& s = shots.back();
+        s.rx.push_back(gx);
+        s.rz.push_back(rec_depth);
+        const std::size_t at = s.traces.size();
+        s.traces.resize(at + static_cast<std::size_t>(ns));
+        decode_segy_samples(format, tr.data() + 240, ns, s.traces.data() + at);
+    }
+    if (shots.empty()) throw std::runtime_error("SEG-Y: no shots decoded");
+    return shots;
+}
+
+std::vector<ShotRecord> read_shots(const std::string& path, float src_depth, float rec_depth) {
     std::string ext;
     const auto dot = path.find_last_of('.');
     if (dot != std::string::npos) ext = path.substr(dot);
     std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return (char)std::tolower(c); });
-    if (ext == ".sgy" || ext == ".segy")
-        throw std::runtime_error("SEG-Y input is not supported by this engine; convert to RTMS");
+    if (ext == ".sgy" || ext == ".segy") return read_shots_segy(path, src_depth, rec_depth);
     return read_shots_raw(path);
 }
 

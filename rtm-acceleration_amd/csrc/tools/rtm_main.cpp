@@ -2,7 +2,7 @@
 //
 // Same flags, defaults, sidecar precedence, checks and outputs as the
 // reference's `rtm` (src/main.cpp:18-257); engines come from make_engine
-// ("hip", "hip-fp16", "hip-bf16").  SEG-Y input is not supported.
+// ("hip", "hip-fp16", "hip-bf16").
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -31,7 +31,7 @@ void usage() {
         "  --gpus N               GPUs (this build: 1)\n\n"
         "Required:\n"
         "  --velocity PATH        float32 velocity model (m/s)\n"
-        "  --shots PATH           shot gathers (RTMS raw)\n"
+        "  --shots PATH           shot gathers (.bin = RTMS raw, .sgy/.segy = SEG-Y)\n"
         "  --output PATH          migrated image, float32, image[ix*nz+iz]\n\n"
         "Velocity geometry (read from PATH.hdr when present; CLI overrides):\n"
         "  --nx N  --nz N  --dx M  --dz M  --ox M  --oz M  --vel-layout zfast|xfast\n"
@@ -61,7 +61,7 @@ int main(int argc, char** argv) try {
     RTMParams par;
     bool set_nx = false, set_nz = false, set_dx = false, set_dz = false, set_ox = false, set_oz = false;
     bool set_layout = false, want_illum = false, want_illum_map = false;
-    float mute_v = 0.0f;
+    float mute_v = 0.0f, segy_sz = 10.0f, segy_rz = 10.0f;
 
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -94,7 +94,8 @@ int main(int argc, char** argv) try {
         else if (a == "--benchmark") bench_path = val();
         else if (a == "--benchmark-csv") bench_csv_path = val();
         else if (a == "--dataset") dataset_label = val();
-        else if (a == "--segy-src-depth" || a == "--segy-rec-depth") (void)val();
+        else if (a == "--segy-src-depth") segy_sz = std::stof(val());
+        else if (a == "--segy-rec-depth") segy_rz = std::stof(val());
         else if (a == "--quiet") par.verbose = false;
         else if (a == "--engine") engine_name = val();
         else if (a == "--gpus") gpus = std::stoi(val());
@@ -133,7 +134,7 @@ int main(int argc, char** argv) try {
         }
     }
     VelocityModel model = io::read_velocity_raw(vel_path, vh);
-    std::vector<ShotRecord> shots = io::read_shots(shots_path, 10.0f, 10.0f);
+    std::vector<ShotRecord> shots = io::read_shots(shots_path, segy_sz, segy_rz);
     if (max_shots > 0 && (int)shots.size() > max_shots) shots.resize(max_shots);
     if (mute_v > 0.0f)
         for (auto& s : shots) mute_direct_wave(s, mute_v, 1.0f / par.f0, 1.0f / par.f0);

@@ -59,3 +59,34 @@ def test_layered3_cli_pipeline(tmp_path):
     assert sha(tmp_path / "image_filtered.bin") == man["sha256"]["filtered_illum.f32"]
     assert csv.read_text().splitlines()[0].count(",") == 25
     run("rtm_compare", os.path.join(d, "image.f32"), str(out), "--nx", "201", "--nz", "151")
+
+
+def test_segy_input_cli(tmp_path):
+    """The layered workload's shots written as SEG-Y (IEEE float samples,
+    record number per shot, integer coordinates, depths from the command
+    line) migrate to the reference image bit for bit, as the RTMS file does."""
+    from test_segy_cpu import write_segy
+    d = os.path.join(GOLD, "layered3")
+    man = json.load(open(os.path.join(d, "manifest.json")))
+    vel = os.path.join(d, "velocity.bin")
+    shots = tmp_path / "shots.bin"
+    run("rtm_synth", "--velocity", vel, "--outdir", str(tmp_path), "--shots-out", str(shots), "--nt", "1400",
+        "--dt", "0.001", "--f0", "12", "--nshots", "4", "--rec-dx", "10")
+    raw = shots.read_bytes()
+    nshots, nrec, nt = np.frombuffer(raw, "<i4", 3, 8)
+    assert np.frombuffer(raw, "<f4", 1, 20)[0] == np.float32(0.001)
+    per = 2 + 2 * nrec + nrec * nt
+    body = np.frombuffer(raw, "<f4", nshots * per, 32).reshape(nshots, per)
+    traces = []
+    for k, sh in enumerate(body):
+        sx, sz, rx, rz = sh[0], sh[1], sh[2:2 + nrec], sh[2 + nrec:2 + 2 * nrec]
+        assert sz == 20.0 and np.all(rz == 20.0) and sx == np.round(sx) and np.all(rx == np.round(rx))
+        tr = sh[2 + 2 * nrec:].reshape(nrec, nt)
+        traces += [(k + 1, 0, int(sx), int(x), tr[r].astype(">f4").tobytes()) for r, x in enumerate(rx)]
+    sgy = tmp_path / "line.sgy"
+    write_segy(sgy, 5, 1000, int(nt), traces)
+    out = tmp_path / "image_sgy.bin"
+    run("rtm", "--engine", "hip", "--velocity", vel, "--shots", str(sgy), "--segy-src-depth", "20",
+        "--segy-rec-depth", "20", "--output", str(out), "--order", "8", "--nb", "60", "--f0", "12",
+        "--store-interval", "2", "--mute-direct", "1500", "--quiet")
+    assert sha(out) == man["sha256"]["image.f32"]
