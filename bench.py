@@ -111,11 +111,34 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
         prop = f + b
     wall = time.perf_counter() - t0
     cells = 2.0 * nt_sample * nxe * nze
-    return {"value": round(cells / wall * 1e-9, 4), "unit": "Gcell-steps/s", "cores": int(threads),
-            "kind": kind,
-            "sample": f"cpu-opt engine, 1 shot of the 2301x751 workload, nt={nt_sample} of {shot.nt} "
-                      f"({cells * 1e-9:.2f} G cell-steps, {wall:.1f} s wall, {prop:.1f} s propagation), "
-                      f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
+    out = {"value": round(cells / wall * 1e-9, 4), "unit": "Gcell-steps/s", "cores": int(threads),
+           "kind": kind,
+           "sample": f"cpu-opt engine, 1 shot of the 2301x751 workload, nt={nt_sample} of {shot.nt} "
+                     f"({cells * 1e-9:.2f} G cell-steps, {wall:.1f} s wall, {prop:.1f} s propagation), "
+                     f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
+    if kind == "reference":
+        out["cpu_engine"] = cpu_single_thread(model, params, dt, shot, budget_s / 2)
+    return out
+
+
+def cpu_single_thread(model, params, dt, shot, budget_s: float):
+    """The reference's plain cpu engine (one thread, -O2) on the same shot,
+    nt reduced to a budget of about budget_s seconds."""
+    import oracle_lib as O
+    nxe, nze = model.nx + 2 * params.nb, model.nz + 2 * params.nb
+    nt_probe = 4
+    t0 = time.perf_counter()
+    O.ref_migrate("cpu", model, params, nt_probe, dt, [shot])
+    probe = time.perf_counter() - t0
+    nt_sample = int(max(8, min(shot.nt, nt_probe * budget_s / max(probe, 1e-3))))
+    t0 = time.perf_counter()
+    O.ref_migrate("cpu", model, params, nt_sample, dt, [shot])
+    wall = time.perf_counter() - t0
+    cells = 2.0 * nt_sample * nxe * nze
+    return {"value": round(cells / wall * 1e-9, 4), "unit": "Gcell-steps/s",
+            "cores": int(O.ref().ref_num_threads(b"cpu")), "kind": "reference",
+            "sample": f"cpu engine, 1 shot of the 2301x751 workload, nt={nt_sample} of {shot.nt} "
+                      f"({cells * 1e-9:.2f} G cell-steps, {wall:.1f} s wall), IEEE subnormals kept"}
 
 
 def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=0, use_graph=True,
