@@ -415,6 +415,13 @@ def main():
     eng.set_kernel_timing(0)
     ks = eng.kernel_stats()
     info = eng.info()
+    # post-processing of the stacked image on the device (outside the timed
+    # region: the reference applies it once per run, after the last shot)
+    eng.postprocess(illum=True, laplacian=True)        # first call sizes its buffer
+    t0 = time.perf_counter()
+    for _ in range(5):
+        eng.postprocess(illum=True, laplacian=True)
+    post_ms = (time.perf_counter() - t0) / 5 * 1e3
     eng.close()
 
     cells = 2.0 * nt * n_ext * K
@@ -451,6 +458,9 @@ def main():
                      "forward_kernel_us": round(fwd_ms * 1e3, 3),
                      "sampled_launches": ks["sampled"]},
         "image_checksum": float(np.abs(img).sum()),
+        "postprocess": {"ms": round(post_ms, 3),
+                        "what": "illumination compensation + Laplacian filter of the stacked image on the "
+                                "device, including the download of the result"},
     }
     if not args.no_batched:
         print("bench: batch engine on the same shots", file=sys.stderr, flush=True)
