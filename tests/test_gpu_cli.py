@@ -90,3 +90,27 @@ def test_segy_input_cli(tmp_path):
         "--segy-rec-depth", "20", "--output", str(out), "--order", "8", "--nb", "60", "--f0", "12",
         "--store-interval", "2", "--mute-direct", "1500", "--quiet")
     assert sha(out) == man["sha256"]["image.f32"]
+
+
+@pytest.mark.parametrize("engine,label,tol", [("hip-fp16", "FP16 snapshots", 1e-3),
+                                              ("hip-bf16", "BF16 snapshots", 1e-2)])
+def test_16bit_engines_cli(tmp_path, engine, label, tol):
+    """rtm --engine hip-fp16 / hip-bf16 (16-bit snapshots with a per-snapshot
+    scale, FP64 stack) on the layered workload: the image stays within the
+    16-bit tolerance of the reference's FP32 image (relative L2; 1e-3 for FP16
+    with its 11-bit significand, 1e-2 for BF16 with 8 bits)."""
+    d = os.path.join(GOLD, "layered3")
+    vel = os.path.join(d, "velocity.bin")
+    shots = tmp_path / "shots.bin"
+    run("rtm_synth", "--velocity", vel, "--outdir", str(tmp_path), "--shots-out", str(shots), "--nt", "1400",
+        "--dt", "0.001", "--f0", "12", "--nshots", "4", "--rec-dx", "10")
+    out = tmp_path / "image.bin"
+    r = run("rtm", "--engine", engine, "--velocity", vel, "--shots", str(shots), "--output", str(out),
+            "--order", "8", "--nb", "60", "--f0", "12", "--store-interval", "2", "--mute-direct", "1500",
+            "--quiet")
+    assert label in r.stdout
+    ref = np.fromfile(os.path.join(d, "image.f32"), np.float32).astype(np.float64)
+    img = np.fromfile(out, np.float32).astype(np.float64)
+    rel = np.linalg.norm(img - ref) / np.linalg.norm(ref)
+    print(f"{engine}: relative L2 to the reference FP32 image {rel:.3g}", flush=True)
+    assert 0 < rel <= tol
