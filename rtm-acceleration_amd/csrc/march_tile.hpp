@@ -160,7 +160,14 @@ __device__ __forceinline__ int entry_cell(const StepArgs& a, const TileEntries& 
 
 // Source / receiver injection and trace recording of one output cell, in
 // entry order (the tile's entries are sorted by cell, then code): entries
-// [lo, hi) of the tile are the ones that may hit this cell.
+// [lo, hi) of the tile are the ones that may hit this cell; cell is -1 in a
+// lane that outputs no cell.
+//
+// Must be called with every lane of the wave active (never inside a
+// lane-dependent branch): the entries are read across lanes with readlane,
+// and a register's value in lanes that are off in a divergent region is not
+// kept by the compiler.  The loop itself has no lane-dependent branch either:
+// the add is a select, and only the recording store is masked.
 template <Dir D>
 __device__ __forceinline__ float apply_entries(const StepArgs& a, const TileEntries& te, int cell, float out,
                                                float v2, int lo, int hi) {
@@ -181,13 +188,13 @@ __device__ __forceinline__ float apply_entries(const StepArgs& a, const TileEntr
             if constexpr (D == Dir::Backward) amp = sld(a.amps + (ptrdiff_t)a.it * a.amp_count + e);
             else amp = en_code < 0 ? sld(a.wavelet + a.it) : 0.0f;
         }
-        if (en_cell != cell) continue;
-        if constexpr (D == Dir::Forward) {
-            if (en_code < 0) out = out + v2 * amp;
-            else gbl(a.rec_out)[(ptrdiff_t)en_code * a.rec_stride + a.it] = out;
-        } else {
-            out = out + v2 * amp;
+        const bool hit = en_cell == cell;
+        if (D == Dir::Forward && en_code >= 0) {          // en_code is wave-uniform
+            if (hit) gbl(a.rec_out)[(ptrdiff_t)en_code * a.rec_stride + a.it] = out;
+            continue;
         }
+        const float inj = out + v2 * amp;
+        out = hit ? inj : out;
     }
     return out;
 }
@@ -417,14 +424,15 @@ __device__ __forceinline__ void store_cell(const StepArgs& a, const SnapCtx& sc,
     }
 }
 
-// One output cell with its injection entries [e0, e1), applied per cell.
+// One cell of column x with its injection entries [e0, e1), applied per
+// cell with the whole wave active; only the store is masked to output lanes.
 template <Dir D, bool S16>
-__device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_int, int jz,
-                                            float pc, float lap, const ColumnOps& o, float wzv,
+__device__ __forceinline__ void finish_cell(const StepArgs& a, const SnapCtx& sc, int x, int z, bool x_out,
+                                            bool z_int, int jz, float pc, float lap, const ColumnOps& o, float wzv,
                                             const TileEntries& te, int e0, int e1, float& vmax) {
     float out = leapfrog(pc, lap, o, wzv);
-    out = apply_entries<D>(a, te, x * a.ldz + z, out, o.vv, e0, e1);
-    store_cell<D, S16>(a, sc, x, z, z_int, jz, out, o, vmax);
+    out = apply_entries<D>(a, te, x_out ? x * a.ldz + z : -1, out, o.vv, e0, e1);
+    if (x_out) store_cell<D, S16>(a, sc, x, z, z_int, jz, out, o, vmax);
 }
 
 // Entry (en_cell, en_code, amp) on the TX output cells of a lane (cell[r]:
@@ -664,8 +672,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs& a, const Coeffs& cf,
             const long long col_end = (long long)(x + 1) * a.ldz;
             while (ee < e1 && entry_cell(a, te, ee) < col_end) ++ee;
             const float lap = laplacian<H, true>(cf, q, r + H);
-            if (z_out && x < a.nxe - H)
-                finish_cell<D, S16>(a, sc, x, z, z_int, jz, q[r + H], lap, ops[r], wzv, te, ec, ee, vmax);
+            finish_cell<D, S16>(a, sc, x, z, z_out && x < a.nxe - H, z_int, jz, q[r + H], lap, ops[r], wzv, te, ec,
+                                ee, vmax);
             ec = ee;
         }
         if (k + 1 < NCH) {

@@ -89,8 +89,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 #pragma unroll
     for (int r = 0; r < RX; ++r) {
         const int ix = x0 + lx0 + r;
-        if (!z_ok || ix < H || ix >= a.nxe - H) continue;
+        const bool ok = z_ok && ix >= H && ix < a.nxe - H;
 
+        // every lane computes (the operands of non-output lanes are zeros
+        // or in-tile LDS values) and takes part in the entry pass, whose
+        // entries are read across lanes; only the stores are masked
         const float* T = tile + (lx0 + r + H) * LZ + H + lane;
         const float pc = q[r + H];
         float lap = a.coef.c0 * pc;
@@ -106,7 +109,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         const float prev = pv[r] * w;
         float out = (2.0f * pc - prev) + vv[r] * lap;
         out = out * w;
-        out = apply_entries<D>(a, te, (int)i, out, vv[r], e0, e1);
+        out = apply_entries<D>(a, te, ok ? (int)i : -1, out, vv[r], e0, e1);
+        if (!ok) continue;
         a.p_next[i] = out;
 
         const int jx = ix - a.nb;
