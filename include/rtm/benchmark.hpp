@@ -57,4 +57,9 @@ std::string date_now_iso();
 void append_benchmark_csv(const std::string& path, const BenchmarkContext& ctx,
                           const EngineResult& result, const RunInfo& info);
 
+// Appends one row of comma-joined fields to the CSV at `path`, writing
+// `header` first when the file is new or empty (the benchmark and the
+// comparison CSVs both use it).
+void append_csv_row(const std::string& path, const std::string& header, const std::vector<std::string>& fields);
+
 }  // namespace rtm

@@ -24,17 +24,21 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     const int fwd = total < J.nt;
     const int s = fwd ? total : total - J.nt - (wide ? 0 : 1);
     const int base = fwd ? 0 : J.bwd_base;
-    a.p_prev = rotate(J, (base + s) % 3);
-    a.p_cur = rotate(J, (base + s + 1) % 3);
-    a.p_next = rotate(J, (base + s + 2) % 3);
+    // a phase's first steps read p^{n-1} (steps 0, 1) and p^n (step 0) as
+    // zero: from the model's zero field, so the march tile loads them as any
+    // other field (the flags serve the other mappings)
     a.zero_prev = s <= 1;
     a.zero_cur = s == 0;
+    a.p_prev = a.zero_prev ? J.zero : rotate(J, (base + s) % 3);
+    a.p_cur = a.zero_cur ? J.zero : rotate(J, (base + s + 1) % 3);
+    a.p_next = rotate(J, (base + s + 2) % 3);
     a.coef = J.coef;
     a.vdt2 = J.vdt2;
     a.wx = J.wx;
     a.wz = J.wz;
     a.nxe = J.nxe; a.nze = J.nze; a.ldz = J.ldz; a.nb = J.nb; a.nx = J.nx; a.nz = J.nz;
     a.ntz = J.ntz; a.ntx = J.ntx; a.ntiles = J.ntiles;
+    a.ntz_magic = J.ntz_magic;
     const int it = fwd ? s : J.nt - 1 - s;
     a.it = it;
     a.tile_ptr = fwd ? J.tile_ptr[0] : J.tile_ptr[1];

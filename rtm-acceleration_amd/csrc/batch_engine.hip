@@ -78,6 +78,7 @@ struct Model {
     rtm_hip::Coeffs coef{};
     bool resident = false;
     float *vdt2 = nullptr, *wx = nullptr, *wz = nullptr, *wavelet = nullptr;
+    float* zero = nullptr;        // a field of zeros: p^{n-1}, p^n of a phase's first steps
 };
 
 // One submitted shot migration and, once admitted, its device buffers.
@@ -229,17 +230,19 @@ int model_init(rtm_hip_batch* b, Model& m, const rtm_hip_grid* grid, const float
     m.nze = m.grid.nz + 2 * m.params.nb;
     m.ldz = round_up(m.nze, rtm_hip::kTileZ);
     if ((long long)m.nxe * m.ldz >= (1LL << 31)) return b->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
+    if (!rtm_hip::step_grid_ok(m.nxe, m.nze, m.ldz, m.grid.nx, m.grid.nz))
+        return b->fail(RTM_HIP_ERR_ARG, "grid too large for the step kernels (a field or the image exceeds 2 GiB)");
     m.nsnap = (m.nt - 1) / m.params.store_interval + 1;
     m.n_int = (size_t)m.grid.nx * m.grid.nz;
-    m.n_fld = (size_t)m.nxe * m.ldz;
-    m.bytes = (m.n_fld + (size_t)m.nxe + (size_t)m.ldz + (size_t)m.nt) * sizeof(float);
+    m.n_fld = (size_t)(m.nxe + rtm_hip::kGuardCols) * m.ldz;   // guard columns of zeros (kGuardCols)
+    m.bytes = (2 * m.n_fld + (size_t)m.nxe + rtm_hip::kGuardCols + (size_t)m.ldz + (size_t)m.nt) * sizeof(float);
     m.v.assign(v, v + m.n_int);
     return RTM_HIP_OK;
 }
 
 void model_release(rtm_hip_batch* b, Model& m) {
-    for (float* p : {m.vdt2, m.wx, m.wz, m.wavelet}) b->dev_free(p);
-    m.vdt2 = m.wx = m.wz = m.wavelet = nullptr;
+    for (float* p : {m.vdt2, m.wx, m.wz, m.wavelet, m.zero}) b->dev_free(p);
+    m.vdt2 = m.wx = m.wz = m.wavelet = m.zero = nullptr;
     m.resident = false;
 }
 
@@ -264,6 +267,12 @@ int model_upload(rtm_hip_batch* b, Model& m) {
             model_release(b, m);
             return b->hip_fail(e, "model tables");
         }
+    }
+    hipError_t ez = b->dev_alloc(reinterpret_cast<void**>(&m.zero), m.n_fld * sizeof(float));
+    if (ez == hipSuccess) ez = hipMemset(m.zero, 0, m.n_fld * sizeof(float));
+    if (ez != hipSuccess) {
+        model_release(b, m);
+        return b->hip_fail(ez, "model zero field");
     }
     m.resident = true;
     m.v.clear();
@@ -414,6 +423,7 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
                                                j.desc.amps, b->stream);
     if (e != hipSuccess) return b->hip_fail(e, "amplitude table");
     j.desc.ntz = g.ntz;
+    j.desc.ntz_magic = g.ntz_magic;
     j.desc.ntx = g.ntx;
     j.desc.ntiles = g.ntiles;
     j.tile_x_built = tile_x;
@@ -433,6 +443,7 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     d.n_int = (long long)m.n_int;
     d.bwd_base = (m.nt + 1) % 3;            // buffer of the forward phase's last field
     d.vdt2 = m.vdt2; d.wx = m.wx; d.wz = m.wz; d.wavelet = m.wavelet;
+    d.zero = m.zero;
     float *tr = nullptr, *img = nullptr, *ill = nullptr;
     int rc = job_alloc(b, j.allocs, &tr, j.traces.size(), "traces");
     float* amps = nullptr;

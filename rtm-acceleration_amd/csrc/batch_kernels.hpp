@@ -21,6 +21,7 @@ struct JobDesc {
     int nxe, nze, ldz, nb, nx, nz, half;
     int nt, store, snap_fmt;
     int ntz, ntx, ntiles;        // this job's tile grid (march mapping)
+    unsigned ntz_magic;          // tile_div_magic(ntz)
     int tile_first;              // its first tile in the group launch
     int bwd_base;                // buffer rotation base of the backward phase
     // clock, advanced on the device after every run of ticks
@@ -36,6 +37,7 @@ struct JobDesc {
     float* amps;                 // receiver amplitudes in backward-entry order (StepArgs::amps)
     int amp_count;
     float* fld[3];
+    const float* zero;           // a field of zeros (the model's), read as p^{n-1} / p^n on a phase's first steps
     float* snap32;               // FP32 snapshots, or
     uint16_t* snap16;            // 16-bit codes +
     float* snap_max;             // per-snapshot max|p|

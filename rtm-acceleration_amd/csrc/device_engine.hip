@@ -399,6 +399,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     a.coef = h->coef;
     a.nxe = h->nxe; a.nze = h->nze; a.ldz = h->ldz; a.nb = h->nb; a.nx = h->nx; a.nz = h->nz;
     a.ntz = g.ntz;
+    a.ntz_magic = g.ntz_magic;
     a.ntx = g.ntx;
     a.ntiles = g.ntiles;
     a.tile_ptr = h->d_tile_ptr[k];
@@ -538,7 +539,7 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     // launches (the minimum is the least noisy estimate of a kernel's time)
     constexpr int kWarm = 4, kRounds = 3, kTimed = 16;
     auto time_geometry = [&](const rtm_hip::StepGeometry& g) -> float {
-        a.ntz = g.ntz; a.ntx = g.ntx; a.ntiles = g.ntiles;
+        a.ntz = g.ntz; a.ntz_magic = g.ntz_magic; a.ntx = g.ntx; a.ntiles = g.ntiles;
         float cand_ms = 1e30f;
         int n = 0;
         for (int round = 0; round < kRounds && !rc; ++round) {
@@ -765,6 +766,8 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     const long long nxe = (long long)g->nx + 2LL * p->nb, nze = (long long)g->nz + 2LL * p->nb;
     const long long ldz = round_up((int)nze, rtm_hip::kTileZ);
     if (nxe * ldz >= (1LL << 31)) return h->fail(RTM_HIP_ERR_ARG, "extended grid exceeds 2^31 cells");
+    if (!rtm_hip::step_grid_ok((int)nxe, (int)nze, (int)ldz, g->nx, g->nz))
+        return h->fail(RTM_HIP_ERR_ARG, "grid too large for the step kernels (a field or the image exceeds 2 GiB)");
 
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
@@ -779,7 +782,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     h->image_stack = p->image_stack;
     h->nsnap = (nt - 1) / p->store_interval + 1;
     h->n_int = (size_t)g->nx * g->nz;
-    h->n_fld = (size_t)nxe * ldz;
+    h->n_fld = (size_t)(nxe + rtm_hip::kGuardCols) * ldz;   // guard columns of zeros (kGuardCols)
     h->snap_bytes = (size_t)h->nsnap * h->n_int *
                     (h->snap_fmt == RTM_HIP_SNAP_F32 ? sizeof(float) : sizeof(uint16_t));
     h->times = rtm_hip_times{};
@@ -807,7 +810,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
 
     int rc = RTM_HIP_OK;
     if (!rc) rc = h->alloc(&h->d_vdt2, h->n_fld, "vdt2");
-    if (!rc) rc = h->alloc(&h->d_wx, (size_t)h->nxe, "wx");
+    if (!rc) rc = h->alloc(&h->d_wx, (size_t)h->nxe + rtm_hip::kGuardCols, "wx");
     if (!rc) rc = h->alloc(&h->d_wz, (size_t)h->ldz, "wz");
     if (!rc) rc = h->alloc(&h->d_wavelet, (size_t)nt, "wavelet");
     for (int b = 0; b < 3 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");

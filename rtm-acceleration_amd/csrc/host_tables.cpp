@@ -28,7 +28,7 @@ HostTables build_tables(int nx, int nz, float dx, float dz, int nb, int order, f
 
     // (v dt)^2 with edge replication into the pad (src/rtm/rtm_cpu.cpp:33-42)
     const float dt2 = dt * dt;
-    t.vdt2.assign(static_cast<std::size_t>(t.nxe) * ldz, 0.0f);
+    t.vdt2.assign(static_cast<std::size_t>(t.nxe + kGuardCols) * ldz, 0.0f);
     for (int ixe = 0; ixe < t.nxe; ++ixe) {
         const int ix = std::clamp(ixe - nb, 0, nx - 1);
         float* col = t.vdt2.data() + static_cast<std::size_t>(ixe) * ldz;
@@ -39,7 +39,7 @@ HostTables build_tables(int nx, int nz, float dx, float dz, int nb, int order, f
         }
     }
 
-    t.wx.resize(t.nxe);
+    t.wx.assign(t.nxe + kGuardCols, 0.0f);
     for (int i = 0; i < t.nxe; ++i) t.wx[i] = cerjan(i, t.nxe, nb, alpha);
     t.wz.assign(ldz, 0.0f);
     for (int i = 0; i < t.nze; ++i) t.wz[i] = cerjan(i, t.nze, nb, alpha);

@@ -7,8 +7,8 @@ namespace rtm_hip {
 
 struct HostTables {
     int nxe = 0, nze = 0, ldz = 0;
-    std::vector<float> vdt2;     // [nxe][ldz], (v dt)^2, 0 in the pitch pad
-    std::vector<float> wx;       // [nxe]
+    std::vector<float> vdt2;     // [nxe + kGuardCols][ldz], (v dt)^2, 0 in the pitch pad and guard
+    std::vector<float> wx;       // [nxe + kGuardCols], 0 in the guard
     std::vector<float> wz;       // [ldz], 0 in the pitch pad
     std::vector<float> wavelet;  // [nt]
     Coeffs coef{};

@@ -44,11 +44,13 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 // Value held by lane+1 / lane-1 of the wave (DPP wave_shl:1 / wave_shr:1, a
 // VALU move instead of an LDS-crossbar permute).  Lanes with no source read 0;
 // they are never output lanes for shifts up to H.
+// bound_ctrl: a lane with no source reads 0 without an old value to merge, so
+// a shift is one DPP move with no initialising move before it.
 __device__ __forceinline__ float from_lane_above(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_lane_below(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
 // max over the wave's 64 lanes, in every lane.  DPP moves (no LDS round
@@ -144,6 +146,16 @@ __device__ __forceinline__ TileEntries load_entries(const StepArgs& a, int e0, i
         if constexpr (D == Dir::Backward) te.amp = gbl(a.amps)[(ptrdiff_t)a.it * a.amp_count + e0 + lane];
     }
     return te;
+}
+
+// v, computed here with the whole wave active: an empty asm that reads and
+// writes the register keeps the compiler from sinking the arithmetic into
+// the masked store block after it (there, each column would wait again for
+// every earlier load and store of the wave, the previous column's store
+// included).
+__device__ __forceinline__ float settle(float v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 // Waits, once and before any store of the tile, for the entry registers:
@@ -543,69 +555,195 @@ __device__ __forceinline__ void settle_scales(const StepArgs& a, const SnapCtx& 
 }
 
 // ---------------------------------------------------------------------------
+// Strip addressing of the march mapping.  A wave's 64 lanes hold depth cells
+// z0 .. z0+63 of the columns of its tile.  A column's address is
+// wave-uniform (scalar arithmetic) and each lane adds its own offset, fixed
+// for the wave: the vector unit, the step's binding resource, spends one
+// address add per access and nothing on bounds.  Lane offsets are clamped to
+// the last row of their array and interior columns to the last interior
+// column; field columns need no clamping, since every field, (v dt)^2 and the
+// x taper end with kGuardCols columns of zeros, past which no tile's window
+// reaches.  A clamped lane or column, and a guard column, only ever feed
+// cells that are not outputs.  (Buffer instructions, with the column in a
+// scalar offset and no address arithmetic at all, measured slower on this
+// chip than global loads and stores.)
+// ---------------------------------------------------------------------------
+struct Strip {
+    int z0, x0;             // first depth cell of the strip, first output column
+    int z, jz;              // this lane's depth cell and interior depth index
+    int lz;                 // lane offset in a field column: min(lane, nze-1-z0)
+    int jzc;                // interior depth index clamped to [0, nz)
+    bool z_out, z_int;      // output lane; output lane inside the interior
+};
+
+// t must be wave-uniform (a read-first-lane value or one derived from it).
+template <int H>
+__device__ __forceinline__ Strip make_strip(const StepArgs& a, int t, int lane, int tx_cols) {
+    constexpr int TZ = kTileZ - 2 * H;
+    Strip s;
+    // scalar arithmetic: t / ntz by the host's reciprocal (exact for every
+    // tile of the launch)
+    const int tx = a.ntz == 1 ? t : (int)__umulhi((unsigned)t, a.ntz_magic);
+    const int tz = t - tx * a.ntz;
+    s.z0 = tz * TZ;
+    s.x0 = H + tx * tx_cols;
+    s.z = s.z0 + lane;
+    s.jz = s.z - a.nb;
+    s.z_out = lane >= H && lane < kTileZ - H && s.z < a.nze - H;
+    s.z_int = s.z_out && s.jz >= 0 && s.jz < a.nz;
+    s.lz = min(lane, a.nze - 1 - s.z0);
+    s.jzc = min(max(s.jz, 0), a.nz - 1);
+    return s;
+}
+
+// Column x (x < nxe + kGuardCols) of a field at the strip's first row.
+template <typename T>
+__device__ __forceinline__ global_t<T>* field_col(T* p, const StepArgs& a, const Strip& s, int x) {
+    return gbl(p) + ((ptrdiff_t)x * a.ldz + s.z0);
+}
+// Interior column jx, clamped to [0, nx), of an interior array.
+template <typename T>
+__device__ __forceinline__ global_t<T>* interior_col(T* p, const StepArgs& a, int jx) {
+    return gbl(p) + (ptrdiff_t)min(max(jx, 0), a.nx - 1) * a.nz;
+}
+
+// The snapshot value of stored bits fb (a 16-bit code, an FP32 value, or a
+// pending field's FP32 value encoded and decoded here, which gives the value
+// the 16-bit store would hold).
+template <bool S16>
+__device__ __forceinline__ float strip_snapshot(const StepArgs& a, const SnapCtx& c, uint32_t fb) {
+    if constexpr (!S16) return __uint_as_float(fb);
+    const uint32_t code = a.pending_field ? encode_snapshot(__uint_as_float(fb), c.k_in, a.snap_fmt) : fb;
+    return decode_code(code, c.k_in, a.snap_fmt);
+}
+
+// The imaging operands of output columns x0 .. x0+TX-1 of a backward step
+// that images: the snapshot bits, the image and the illumination.  One
+// uniform branch around all of them (a branch per column between the tile's
+// loads lets the compiler delay the loads behind it).  Columns outside the
+// interior read a clamped column and are never imaged.
+template <int TX, bool S16>
+__device__ __forceinline__ void strip_imaging_ops(const StepArgs& a, const Strip& s, ColumnOps (&o)[TX]) {
+#pragma unroll
+    for (int r = 0; r < TX; ++r) {
+        const int jx = s.x0 + r - a.nb;
+        if constexpr (S16) {
+            // a 16-bit code, or (first backward step of a batched job) the
+            // FP32 field still pending its encoding: both loads are from
+            // valid memory (the unused one from the image), one is kept
+            const uint16_t* codes = a.pending_field ? (const uint16_t*)a.image : a.snap16_in;
+            const float* pend = a.pending_field ? a.pending_field : a.p_cur;
+            const uint32_t code = __builtin_nontemporal_load(interior_col(codes, a, jx) + s.jzc);
+            const uint32_t pv = __float_as_uint(field_col(pend, a, s, s.x0 + r)[s.lz]);
+            o[r].fb = a.pending_field ? pv : code;
+        } else {
+            o[r].fb = __float_as_uint(__builtin_nontemporal_load(interior_col(a.snap_in, a, jx) + s.jzc));
+        }
+        o[r].iv = interior_col(a.image, a, jx)[s.jzc];
+        o[r].sv = interior_col(a.illum, a, jx)[s.jzc];
+    }
+}
+
+// p^{n+1} of output cell (x, lane) to the field, and its snapshot / imaging
+// epilogue (src/rtm/forward.cpp:112-118; src/rtm/backward.cpp:66-75).  The
+// caller has checked that the lane and the column are outputs.
+template <Dir D, bool S16>
+__device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc, const Strip& s, int x, float out,
+                                            const ColumnOps& o, float& vmax) {
+    field_col(a.p_next, a, s, x)[s.lz] = out;
+    const int jx = x - a.nb;
+    if (jx < 0 || jx >= a.nx) return;                       // wave-uniform
+    if (!s.z_int) return;
+    const ptrdiff_t j = (ptrdiff_t)jx * a.nz + s.jzc;
+    if constexpr (D == Dir::Forward) {
+        if constexpr (S16) {
+            if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
+            if (sc.quantise && !a.dry)
+                __builtin_nontemporal_store(encode_snapshot(o.pv, sc.k_q, a.snap_fmt), gbl(a.quant_out) + j);
+        } else {
+            if (sc.store32 && !a.dry) __builtin_nontemporal_store(out, gbl(a.snap_out) + j);
+        }
+    } else {
+        if (sc.imaging && !a.dry) {
+            const float fv = strip_snapshot<S16>(a, sc, o.fb);
+            gbl(a.image)[j] = o.iv + fv * out;
+            gbl(a.illum)[j] = o.sv + fv * fv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
 // LEAN: a step with no snapshot work (a backward step that does not image,
 // an FP32 forward step that stores no snapshot), built without that
 // epilogue: a smaller kernel for three steps in four.
+// t must be wave-uniform (a value read-first-lane'd or derived from one).
 template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
 __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
-    constexpr int TZ = kTileZ - 2 * H;       // output depth cells per strip
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
-
-    const int tz = t % a.ntz, tx = t / a.ntz;
-    const int z = tz * TZ + lane;            // depth cell held by this lane
-    const int x0 = H + tx * TX;              // first output column
-    const bool z_load = z < a.nze;
-    const bool z_out = lane >= H && lane < kTileZ - H && z < a.nze - H;
-    const int jz = z - a.nb;                 // interior depth index
-    const bool z_int = z_out && jz >= 0 && jz < a.nz;
+    const Strip s = make_strip<H>(a, t, lane, TX);
+    const int x0 = s.x0;
     SnapCtx sc = snap_ctx<S16>(a, D);
     if constexpr (LEAN) sc.imaging = sc.store32 = false;
     snap_prefetch<S16>(a, sc, lane);
     // The entry bounds are scalar loads: issued before the tile's loads they
-    // are in flight with them (FP32 steps 1-2 % faster); the 16-bit build
-    // has its slice-max load there already and does better without them.
+    // are in flight with them; the 16-bit build has its slice-max load there
+    // already and does better without them.
     int e0 = 0, e1 = 0;
     if constexpr (!S16) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
 
+    // every load of the tile, issued before any use
+    const global_t<const float>* pc = field_col(a.p_cur, a, s, x0 - H);
     float q[NQ];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) q[j] = load_cur(a, x0 - H + j, z, z_load);
+    for (int j = 0; j < NQ; ++j) q[j] = pc[(ptrdiff_t)j * a.ldz + s.lz];
     ColumnOps ops[TX];
+    const global_t<const float>* pp = field_col(a.p_prev, a, s, x0);
+    const global_t<const float>* pv2 = field_col(a.vdt2, a, s, x0);
 #pragma unroll
-    for (int r = 0; r < TX; ++r) ops[r] = column_ops<H, D, S16>(a, sc, x0 + r, z, z_out, z_int, jz);
-    const float wzv = z_load ? gbl(a.wz)[z] : 0.0f;
+    for (int r = 0; r < TX; ++r) {
+        ops[r].pv = pp[(ptrdiff_t)r * a.ldz + s.lz];
+        ops[r].vv = pv2[(ptrdiff_t)r * a.ldz + s.lz];
+        ops[r].wx = sld(a.wx + x0 + r);
+        ops[r].iv = ops[r].sv = 0.0f;
+        ops[r].fb = 0;
+    }
+    if constexpr (D == Dir::Backward && !LEAN)
+        if (sc.imaging) strip_imaging_ops<TX, S16>(a, s, ops);
+    const float wzv = gbl(a.wz)[s.z0 + s.lz];
     if constexpr (S16) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
     snap_scales<S16>(a, sc, lane);
-    entries_ready(te);
     float vmax = 0.0f;
 
-    if (e0 == e1) {
-        // no injection entries (nearly every tile): each cell is stored as
-        // soon as it is computed, which keeps the stores streaming
+    // One pass over the columns computes every output once.  A tile without
+    // injection entries (nearly every tile) stores each cell as soon as it is
+    // computed, which keeps the stores streaming; a tile with entries keeps
+    // its outputs, applies the entries in one pass, then stores.  (Two
+    // separate column loops, one per case, made the compiler hoist their
+    // common sums above the branch and hold every column live at once.)  No
+    // early exit past the last column: only the stores are guarded, so no
+    // branch stands between the tile's loads.
+    const bool plain = e0 == e1;                            // wave-uniform
+    float out[TX];
 #pragma unroll
-        for (int r = 0; r < TX; ++r) {
-            const float lap = laplacian<H, DPP>(cf, q, r + H);
-            const int x = x0 + r;
-            if (!z_out || x >= a.nxe - H) continue;
-            store_cell<D, S16>(a, sc, x, z, z_int, jz, leapfrog(q[r + H], lap, ops[r], wzv), ops[r], vmax);
-        }
-    } else {
-        float out[TX];
+    for (int r = 0; r < TX; ++r) {
+        out[r] = settle(leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv));
+        const int x = x0 + r;
+        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S16>(a, sc, s, x, out[r], ops[r], vmax);
+    }
+    if (!plain) {
+        // a tile with entries waits for its entry registers once, here
+        entries_ready(te);
         int cell[TX];
 #pragma unroll
-        for (int r = 0; r < TX; ++r) {
-            const int x = x0 + r;
-            out[r] = leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv);
-            cell[r] = (z_out && x < a.nxe - H) ? x * a.ldz + z : -1;
-        }
+        for (int r = 0; r < TX; ++r) cell[r] = (s.z_out && x0 + r < a.nxe - H) ? (x0 + r) * a.ldz + s.z : -1;
         apply_tile_entries<D, TX>(a, te, cell, out, ops, e0, e1);
 #pragma unroll
         for (int r = 0; r < TX; ++r)
-            if (cell[r] >= 0) store_cell<D, S16>(a, sc, x0 + r, z, z_int, jz, out[r], ops[r], vmax);
+            if (cell[r] >= 0) strip_store<D, S16>(a, sc, s, x0 + r, out[r], ops[r], vmax);
     }
     finish_tile<S16>(a, sc, vmax, lane, t, a.ntiles);
     settle_scales<S16>(a, sc, t, a.ntiles, lane);

@@ -146,6 +146,7 @@ StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze
         g.ntiles = g.ntz * g.ntx;
         g.blocks = (g.ntiles + kWaves - 1) / kWaves;
     }
+    g.ntz_magic = tile_div_magic(g.ntz);
     return g;
 }
 

@@ -16,6 +16,10 @@ constexpr int kTileZ = 64;        // one wave64 spans 64 consecutive depth cells
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kBlockThreads = kTileZ * kWaves;
 constexpr int kReduceWaves = 64;  // waves of a launch that share a snapshot-max reduction
+// Columns of zeros after the last column of every field, of (v dt)^2 and of
+// the x taper: a march tile's window ends at most tile_x - 1 <= 15 columns past
+// the last column, so its column offsets need no clamping.
+constexpr int kGuardCols = 16;
 
 
 // Stencil coefficients pre-divided by the spacing (src/rtm/rtm_cpu.cpp:75-79).
@@ -52,6 +56,7 @@ struct StepArgs {
     Coeffs       coef;
     int nxe, nze, ldz, nb, nx, nz;
     int ntz, ntx, ntiles;     // tile grid of the launch (StepGeometry)
+    unsigned ntz_magic;       // tile_div_magic(ntz)
     int it;
     // per-tile injection / recording table (CSR over tiles)
     const int*   tile_ptr;    // [ntiles+1]
@@ -142,6 +147,7 @@ struct StepGeometry {
     int x_origin = 0;    // first column of tile 0
     int z_origin = 0;    // first depth cell of tile 0
     int ntz = 0, ntx = 0, ntiles = 0;
+    unsigned ntz_magic = 0;   // tile_div_magic(ntz)
     int blocks = 0;      // workgroups launched
     // occupancy cap: workgroups per CU (0 = none), enforced by reserving
     // lds_bytes of dynamic LDS per workgroup (the march kernels use none)
@@ -155,6 +161,24 @@ struct StepGeometry {
 int occupancy_lds_bytes(int device, int blocks_per_cu);
 
 StepGeometry step_geometry(Family family, int half, int tile_x, int nxe, int nze, int ldz);
+
+// t / ntz for a wave-uniform tile index t, as the march kernels compute it:
+// umulhi(t, magic) with magic = ceil(2^32 / ntz), exact whenever t * ntz < 2^32
+// (t < ntz * ntx); ntz == 1 needs no division and has magic 0.
+inline unsigned tile_div_magic(int ntz) {
+    return ntz <= 1 ? 0u : (unsigned)((0x100000000ull + (unsigned)ntz - 1) / (unsigned)ntz);
+}
+
+// Whether the step kernels can address an extended grid of nxe x ldz cells
+// and an interior of nx x nz: every field and interior array fits a 31-bit
+// byte range (buffer resources), and the tile index times the strip count
+// stays below 2^32 for the narrowest tiles (tile_div_magic).
+inline bool step_grid_ok(int nxe, int nze, int ldz, int nx, int nz) {
+    const unsigned long long lim = 1ull << 31;
+    if ((unsigned long long)(nxe + kGuardCols) * ldz * 4 >= lim || (unsigned long long)nx * nz * 4 >= lim) return false;
+    const unsigned long long ntz = (unsigned long long)nze / 50 + 1;   // strips of >= 52 cells
+    return ntz * ntz * (unsigned long long)nxe < (1ull << 32);
+}
 
 // Tile widths each family is instantiated for.  The odd widths of the DPP
 // march let the tile count of a grid fall just under a multiple of the

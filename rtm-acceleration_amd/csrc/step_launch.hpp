@@ -140,7 +140,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 // March mapping: one wave per tile (march_tile.hpp).
 template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
-    const int wave = (int)threadIdx.x / kTileZ;
+    // the wave index is wave-uniform: the tile's coordinates and column
+    // offsets are then scalar arithmetic
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;

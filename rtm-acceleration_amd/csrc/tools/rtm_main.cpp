@@ -1,8 +1,15 @@
 // rtm -- 2D acoustic reverse time migration on one MI355X.
 //
-// Same flags, defaults, sidecar precedence, checks and outputs as the
-// reference's `rtm` (src/main.cpp:18-257); engines come from make_engine
-// ("hip", "hip-fp16", "hip-bf16").
+// The command line of the reference's `rtm` (flags and defaults of
+// src/main.cpp:18-52): the velocity geometry comes from a flag, else from
+// --vel-header, else from <velocity>.hdr; a time step beyond the CFL limit is
+// refused and fewer than 3 points per wavelength draw a warning; the outputs
+// are the raw zero-lag cross-correlation image and, on request,
+// <stem>_illummap.bin, <stem>_illum.bin, <stem>_filtered.bin, the benchmark
+// report and one row of the benchmark CSV.  Exit status 0, or 1 with a
+// message on any error.  Engines come from make_engine ("hip", "hip-fp16",
+// "hip-bf16"); the post-processed images are computed on the device from the
+// image the HIP engine keeps there.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -11,228 +18,260 @@
 #include <string>
 #include <vector>
 
+#include "cli.hpp"
 #include "rtm/benchmark.hpp"
-#include "rtm/comparison.hpp"
 #include "rtm/rtm_factory.hpp"
 #include "rtm/rtm_hip_engine.hpp"
-#include "rtm_hip.h"
 #include "rtm/rtm_io.hpp"
+#include "rtm_hip.h"
 
 using namespace rtm;
 
 namespace {
 
-void usage() {
-    std::printf(
-        "rtm -- 2D acoustic Reverse Time Migration (MI355X)\n\n"
-        "Engine:\n"
-        "  --engine NAME          hip | hip-fp16 | hip-bf16      (default hip)\n"
-        "  --list-engines         print the engines in this build and exit\n"
-        "  --gpus N               GPUs (this build: 1)\n\n"
-        "Required:\n"
-        "  --velocity PATH        float32 velocity model (m/s)\n"
-        "  --shots PATH           shot gathers (.bin = RTMS raw, .sgy/.segy = SEG-Y)\n"
-        "  --output PATH          migrated image, float32, image[ix*nz+iz]\n\n"
-        "Velocity geometry (read from PATH.hdr when present; CLI overrides):\n"
-        "  --nx N  --nz N  --dx M  --dz M  --ox M  --oz M  --vel-layout zfast|xfast\n"
-        "  --vel-header PATH      explicit sidecar\n\n"
-        "Modelling / migration parameters:\n"
-        "  --order N (8)  --nb N (60)  --sponge-alpha F (0.0053)  --f0 F (12)\n"
-        "  --store-interval N (1)  --nt N  --max-shots N  --mute-direct V (0 = off)\n\n"
-        "Outputs:\n"
-        "  --illumination  --filter laplacian  --write-illum-map\n"
-        "  --benchmark PATH  --benchmark-csv PATH  --dataset NAME  --quiet\n");
+// Everything the command line asks for.
+struct Request {
+    std::string velocity, shots, output, header, filter = "none";
+    std::string report, csv, dataset, engine = "hip";
+    int gpus = 1, nt = 0, max_shots = 0;
+    io::VelocityHeader geometry;
+    RTMParams params;
+    float mute_velocity = 0.0f, segy_src_depth = 10.0f, segy_rec_depth = 10.0f;
+    bool illum = false, illum_map = false, quiet = false;
+};
+
+void declare(cli::Options& o, Request& r) {
+    o.section("Engine")
+        .text("--engine", r.engine, "hip | hip-fp16 | hip-bf16 (default hip)")
+        .action("--list-engines", [] { for (const auto& n : list_engines()) std::printf("%s\n", n.c_str()); },
+                "print the engines of this build and exit")
+        .integer("--gpus", r.gpus, "GPUs (this build: 1)");
+    o.section("Inputs and output (required)")
+        .text("--velocity", r.velocity, "float32 velocity model, m/s")
+        .text("--shots", r.shots, "shot gathers: .bin = RTMS, .sgy/.segy = SEG-Y")
+        .text("--output", r.output, "migrated image, float32, image[ix*nz+iz]");
+    o.section("Velocity geometry (a flag wins over --vel-header, which wins over PATH.hdr)")
+        .integer("--nx", r.geometry.nx, "samples along x")
+        .integer("--nz", r.geometry.nz, "samples along z")
+        .real("--dx", r.geometry.dx, "spacing along x, m")
+        .real("--dz", r.geometry.dz, "spacing along z, m")
+        .real("--ox", r.geometry.ox, "origin x, m")
+        .real("--oz", r.geometry.oz, "origin z, m")
+        .text("--vel-layout", r.geometry.layout, "zfast | xfast")
+        .text("--vel-header", r.header, "explicit geometry sidecar");
+    o.section("Modelling and migration")
+        .integer("--order", r.params.order, "spatial FD order 2..12 (default 8)")
+        .integer("--nb", r.params.nb, "sponge width, cells (default 60)")
+        .real("--sponge-alpha", r.params.sponge_alpha, "Cerjan coefficient (default 0.0053)")
+        .real("--f0", r.params.f0, "Ricker peak frequency, Hz (default 12)")
+        .integer("--store-interval", r.params.store_interval, "keep the source wavefield every N steps (default 1)")
+        .integer("--nt", r.nt, "truncate the time axis")
+        .integer("--max-shots", r.max_shots, "migrate only the first N shots")
+        .real("--mute-direct", r.mute_velocity, "direct-wave mute velocity, m/s (0 = off)");
+    o.section("Outputs")
+        .flag("--illumination", r.illum, "also write <output>_illum.bin (illumination compensated)")
+        .text("--filter", r.filter, "laplacian: also write <output>_filtered.bin")
+        .flag("--write-illum-map", r.illum_map, "also write <output>_illummap.bin")
+        .text("--benchmark", r.report, "append the benchmark report to PATH")
+        .text("--benchmark-csv", r.csv, "append one benchmark CSV row to PATH")
+        .text("--dataset", r.dataset, "dataset label of the CSV row (default: shots file stem)")
+        .flag("--quiet", r.quiet, "less chatter");
+    o.section("SEG-Y input")
+        .real("--segy-src-depth", r.segy_src_depth, "source depth, m (default 10)")
+        .real("--segy-rec-depth", r.segy_rec_depth, "receiver depth, m (default 10)");
 }
 
-std::string file_stem(const std::string& path) {
-    const auto slash = path.find_last_of('/');
-    std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
-    const auto dot = name.find_last_of('.');
-    return dot == std::string::npos ? name : name.substr(0, dot);
+// Fills the geometry fields no flag set from the sidecar: the explicit
+// --vel-header (which must exist), else <velocity>.hdr when it is there.
+void complete_geometry(Request& r, const cli::Options& opts) {
+    io::VelocityHeader side;
+    if (!r.header.empty()) {
+        if (!io::read_header_file(r.header, side)) throw std::runtime_error("cannot read velocity header: " + r.header);
+    } else if (!io::read_header_file(r.velocity + ".hdr", side)) {
+        return;
+    }
+    using H = io::VelocityHeader;
+    const struct {
+        const char* flag;
+        void (*take)(H&, const H&);
+    } fields[] = {
+        {"--nx", [](H& d, const H& s) { d.nx = s.nx; }},
+        {"--nz", [](H& d, const H& s) { d.nz = s.nz; }},
+        {"--dx", [](H& d, const H& s) { d.dx = s.dx; }},
+        {"--dz", [](H& d, const H& s) { d.dz = s.dz; }},
+        {"--ox", [](H& d, const H& s) { d.ox = s.ox; }},
+        {"--oz", [](H& d, const H& s) { d.oz = s.oz; }},
+        {"--vel-layout", [](H& d, const H& s) { d.layout = s.layout; }},
+    };
+    for (const auto& f : fields)
+        if (!opts.given(f.flag)) f.take(r.geometry, side);
+}
+
+// "dir/marmousi_shots.bin" -> "marmousi_shots"
+std::string stem_of(const std::string& path) {
+    std::string name = path.substr(path.find_last_of('/') + 1);
+    return name.substr(0, name.find_last_of('.'));
+}
+
+// The output path without its extension: the prefix of the extra images.
+std::string output_prefix(const std::string& path) {
+    const auto dot = path.find_last_of('.');
+    return dot == std::string::npos ? path : path.substr(0, dot);
+}
+
+struct Inputs {
+    VelocityModel model;
+    std::vector<ShotRecord> shots;
+    TimeAxis axis;
+};
+
+Inputs load_inputs(const Request& r) {
+    Inputs in;
+    in.model = io::read_velocity_raw(r.velocity, r.geometry);
+    in.shots = io::read_shots(r.shots, r.segy_src_depth, r.segy_rec_depth);
+    if (in.shots.empty()) throw std::runtime_error("no shots in " + r.shots);
+    if (r.max_shots > 0 && (int)in.shots.size() > r.max_shots) in.shots.resize((std::size_t)r.max_shots);
+    if (r.mute_velocity > 0.0f) {
+        const float window = 1.0f / r.params.f0;      // pad and taper of one source period
+        for (ShotRecord& s : in.shots) mute_direct_wave(s, r.mute_velocity, window, window);
+    }
+    // the first shot sets the time axis for all; --nt can only shorten it
+    in.axis.dt = in.shots.front().dt;
+    in.axis.nt = r.nt > 0 ? std::min(r.nt, in.shots.front().nt) : in.shots.front().nt;
+    return in;
+}
+
+// Prints what will run and refuses an unstable time step.
+void preflight(const Request& r, const RTMEngine& engine, const Inputs& in) {
+    const Grid& g = engine.grid();
+    const float dt = in.axis.dt;
+    const float dt_max = max_stable_dt(in.model.vmax(), g, r.params.order);
+    const float ppw = points_per_wavelength(in.model.vmin(), r.params.f0, g);
+    const bool stable = dt <= dt_max;
+    std::printf("\nmodel       %d x %d cells at %.2f x %.2f m, origin (%.1f, %.1f) m, v %.1f .. %.1f m/s\n"
+                "time        nt %d, dt %.6f s, T %.3f s\n"
+                "stencil     order %d, sponge %d cells, Ricker %.2f Hz, snapshot every %d steps (%.1f MiB)\n"
+                "shots       %zu x %d receivers\n"
+                "stability   dt / dt_max = %.3f (dt_max %.6f s): %s\n"
+                "dispersion  %.2f points per shortest wavelength: %s\n\n",
+                g.nx, g.nz, g.dx, g.dz, g.ox, g.oz, in.model.vmin(), in.model.vmax(), in.axis.nt, dt,
+                in.axis.nt * dt, r.params.order, r.params.nb, r.params.f0, r.params.store_interval,
+                engine.snapshot_bytes() / (1024.0 * 1024.0), in.shots.size(), in.shots.front().nrec(), dt / dt_max,
+                dt_max, stable ? "stable" : "*** UNSTABLE ***", ppw, ppw >= 4.0f ? "ok" : "*** DISPERSIVE ***");
+    if (!stable) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "time step dt = %.6g s exceeds the CFL limit %.6g s of order %d: reduce dt, lower the FD "
+                      "order, or coarsen f0/dx",
+                      dt, dt_max, r.params.order);
+        throw std::runtime_error(msg);
+    }
+    if (ppw < 3.0f)
+        std::fprintf(stderr, "WARNING: fewer than 3 points per wavelength; the image will be visibly dispersive.\n");
+}
+
+// The raw image and the extra images asked for.  The compensated and
+// filtered copies come from the device-resident image when the engine is the
+// HIP one, else from the host functions on the downloaded image.
+void write_images(const Request& r, RTMEngine& engine, const std::vector<float>& image,
+                  const std::vector<float>& illum) {
+    const Grid& g = engine.grid();
+    io::write_raw_floats(r.output, image);
+    std::printf("\nwrote %s  (%d x %d float32, image[ix*nz+iz])\n", r.output.c_str(), g.nx, g.nz);
+    const std::string prefix = output_prefix(r.output);
+    if (r.illum_map) io::write_raw_floats(prefix + "_illummap.bin", illum);
+
+    auto* hip = dynamic_cast<HipRTM*>(&engine);
+    auto derived = [&](const char* suffix, bool compensate, bool filter) {
+        std::vector<float> out;
+        if (hip) {
+            hip->postprocess((compensate ? RTM_HIP_POST_ILLUM : 0) | (filter ? RTM_HIP_POST_LAPLACIAN : 0), 1e-4f,
+                             out);
+        } else {
+            out = image;
+            if (compensate) illumination_compensation(out, illum, 1e-4f);
+            if (filter) laplacian_filter(out, g);
+        }
+        io::write_raw_floats(prefix + suffix, out);
+        std::printf("wrote %s%s\n", prefix.c_str(), suffix);
+    };
+    if (r.illum) derived("_illum.bin", true, false);
+    if (r.filter == "laplacian") derived("_filtered.bin", r.illum, true);
+}
+
+void report(const Request& r, const RTMEngine& engine, const Inputs& in) {
+    const Grid& g = engine.grid();
+    BenchmarkContext ctx;
+    ctx.nx = g.nx;
+    ctx.nz = g.nz;
+    ctx.nb = g.nb;
+    ctx.dx = g.dx;
+    ctx.dz = g.dz;
+    ctx.nt = in.axis.nt;
+    ctx.dt = in.axis.dt;
+    ctx.nshots = (int)in.shots.size();
+    ctx.order = r.params.order;
+    ctx.store_interval = r.params.store_interval;
+    ctx.snapshot_bytes = engine.snapshot_bytes();
+    ctx.results.push_back({engine.name(), engine.times, engine.is_gpu()});
+    print_benchmark_report(std::cout, ctx);
+    if (!r.report.empty()) {
+        std::ofstream f(r.report, std::ios::app);
+        print_benchmark_report(f, ctx);
+    }
+    if (!r.csv.empty()) {
+        RunInfo info;
+        info.host = host_name();
+        info.cpu_name = cpu_model_name();
+        info.gpu_name = engine.device_name();
+        info.engine = r.engine;
+        info.dataset = r.dataset.empty() ? stem_of(r.shots) : r.dataset;
+        info.threads = engine.num_threads();
+        info.gpus = engine.num_devices();
+        append_benchmark_csv(r.csv, ctx, ctx.results.back(), info);
+        std::printf("appended benchmark row to %s\n", r.csv.c_str());
+    }
+}
+
+int run(int argc, char** argv) {
+    Request r;
+    cli::Options opts("rtm -- 2D acoustic reverse time migration on one MI355X\n"
+                      "usage: rtm --velocity PATH --shots PATH --output PATH [options]");
+    declare(opts, r);
+    opts.action("--help", [&opts] { opts.usage(stdout); }, "this text");
+    const auto rest = opts.parse(argc, argv);
+    if (opts.stopped()) return 0;
+    if (!rest.empty()) throw std::runtime_error("unknown option: " + rest.front());
+    if (r.velocity.empty() || r.shots.empty() || r.output.empty()) {
+        opts.usage(stdout);
+        throw std::runtime_error("--velocity, --shots and --output are required");
+    }
+    r.params.verbose = !r.quiet;
+
+    auto engine = make_engine(r.engine, r.gpus);
+    Timer io_clock;
+    complete_geometry(r, opts);
+    const Inputs in = load_inputs(r);
+    engine->times.io = io_clock.elapsed();
+
+    engine->setup(in.model, r.params, in.axis);
+    preflight(r, *engine, in);
+
+    std::vector<float> image, illum;
+    std::printf("migration   %s\n", engine->name());
+    engine->migrate(in.shots, image, illum);
+    write_images(r, *engine, image, illum);
+    report(r, *engine, in);
+    return 0;
 }
 
 }  // namespace
 
-int main(int argc, char** argv) try {
-    std::string vel_path, shots_path, out_path, bench_path, hdr_path, filter = "none";
-    std::string engine_name = "hip", bench_csv_path, dataset_label;
-    int gpus = 1, nt_override = 0, max_shots = 0;
-    io::VelocityHeader vh;
-    RTMParams par;
-    bool set_nx = false, set_nz = false, set_dx = false, set_dz = false, set_ox = false, set_oz = false;
-    bool set_layout = false, want_illum = false, want_illum_map = false;
-    float mute_v = 0.0f, segy_sz = 10.0f, segy_rz = 10.0f;
-
-    for (int i = 1; i < argc; ++i) {
-        const std::string a = argv[i];
-        auto val = [&]() -> std::string {
-            if (i + 1 >= argc) throw std::runtime_error("missing value after " + a);
-            return argv[++i];
-        };
-        if (a == "--velocity") vel_path = val();
-        else if (a == "--shots") shots_path = val();
-        else if (a == "--output") out_path = val();
-        else if (a == "--vel-header") hdr_path = val();
-        else if (a == "--nx") { vh.nx = std::stoi(val()); set_nx = true; }
-        else if (a == "--nz") { vh.nz = std::stoi(val()); set_nz = true; }
-        else if (a == "--dx") { vh.dx = std::stof(val()); set_dx = true; }
-        else if (a == "--dz") { vh.dz = std::stof(val()); set_dz = true; }
-        else if (a == "--ox") { vh.ox = std::stof(val()); set_ox = true; }
-        else if (a == "--oz") { vh.oz = std::stof(val()); set_oz = true; }
-        else if (a == "--vel-layout") { vh.layout = val(); set_layout = true; }
-        else if (a == "--order") par.order = std::stoi(val());
-        else if (a == "--nb") par.nb = std::stoi(val());
-        else if (a == "--sponge-alpha") par.sponge_alpha = std::stof(val());
-        else if (a == "--f0") par.f0 = std::stof(val());
-        else if (a == "--store-interval") par.store_interval = std::stoi(val());
-        else if (a == "--nt") nt_override = std::stoi(val());
-        else if (a == "--max-shots") max_shots = std::stoi(val());
-        else if (a == "--mute-direct") mute_v = std::stof(val());
-        else if (a == "--illumination") want_illum = true;
-        else if (a == "--write-illum-map") want_illum_map = true;
-        else if (a == "--filter") filter = val();
-        else if (a == "--benchmark") bench_path = val();
-        else if (a == "--benchmark-csv") bench_csv_path = val();
-        else if (a == "--dataset") dataset_label = val();
-        else if (a == "--segy-src-depth") segy_sz = std::stof(val());
-        else if (a == "--segy-rec-depth") segy_rz = std::stof(val());
-        else if (a == "--quiet") par.verbose = false;
-        else if (a == "--engine") engine_name = val();
-        else if (a == "--gpus") gpus = std::stoi(val());
-        else if (a == "--help" || a == "-h") { usage(); return 0; }
-        else if (a == "--list-engines") {
-            for (const auto& n : list_engines()) std::printf("%s\n", n.c_str());
-            return 0;
-        } else throw std::runtime_error("unknown option: " + a);
+int main(int argc, char** argv) {
+    try {
+        return run(argc, argv);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "\nERROR: %s\n", e.what());
+        return 1;
     }
-    if (vel_path.empty() || shots_path.empty() || out_path.empty()) {
-        usage();
-        throw std::runtime_error("--velocity, --shots and --output are required");
-    }
-
-    // ---- data loading: CLI flag > --vel-header > <velocity>.hdr -----------
-    auto engine = make_engine(engine_name, gpus);
-    Timer io_timer;
-    {
-        io::VelocityHeader side;
-        bool found = false;
-        if (!hdr_path.empty()) {
-            if (!io::read_header_file(hdr_path, side))
-                throw std::runtime_error("cannot read velocity header: " + hdr_path);
-            found = true;
-        } else {
-            found = io::read_header_file(vel_path + ".hdr", side);
-        }
-        if (found) {
-            if (!set_nx) vh.nx = side.nx;
-            if (!set_nz) vh.nz = side.nz;
-            if (!set_dx) vh.dx = side.dx;
-            if (!set_dz) vh.dz = side.dz;
-            if (!set_ox) vh.ox = side.ox;
-            if (!set_oz) vh.oz = side.oz;
-            if (!set_layout) vh.layout = side.layout;
-        }
-    }
-    VelocityModel model = io::read_velocity_raw(vel_path, vh);
-    std::vector<ShotRecord> shots = io::read_shots(shots_path, segy_sz, segy_rz);
-    if (max_shots > 0 && (int)shots.size() > max_shots) shots.resize(max_shots);
-    if (mute_v > 0.0f)
-        for (auto& s : shots) mute_direct_wave(s, mute_v, 1.0f / par.f0, 1.0f / par.f0);
-    engine->times.io = io_timer.elapsed();
-
-    TimeAxis ta;
-    ta.nt = nt_override > 0 ? std::min(nt_override, shots[0].nt) : shots[0].nt;
-    ta.dt = shots[0].dt;
-
-    // ---- setup + checks ------------------------------------------------------
-    engine->setup(model, par, ta);
-    const Grid& g = engine->grid();
-    const float dt_max = max_stable_dt(model.vmax(), g, par.order);
-    const float ppw = points_per_wavelength(model.vmin(), par.f0, g);
-    std::printf("\n=== MODEL ===\n"
-                "  nx = %d, nz = %d, dx = %.2f m, dz = %.2f m, origin = (%.1f, %.1f)\n"
-                "  vmin = %.1f m/s, vmax = %.1f m/s\n"
-                "  nt = %d, dt = %.6f s, T = %.3f s\n"
-                "  order = %d, nb = %d, f0 = %.2f Hz, store_interval = %d\n"
-                "  shots = %zu, receivers/shot = %d\n"
-                "  CFL: dt = %.6f s, dt_max = %.6f s (ratio %.3f) %s\n"
-                "  Dispersion: %.2f points per shortest wavelength %s\n"
-                "  Snapshot memory: %.1f MiB\n\n",
-                g.nx, g.nz, g.dx, g.dz, g.ox, g.oz, model.vmin(), model.vmax(), ta.nt, ta.dt, ta.nt * ta.dt,
-                par.order, par.nb, par.f0, par.store_interval, shots.size(), shots[0].nrec(), ta.dt, dt_max,
-                ta.dt / dt_max, ta.dt <= dt_max ? "OK" : "*** UNSTABLE ***", ppw,
-                ppw >= 4.0f ? "OK" : "*** DISPERSIVE ***", engine->snapshot_bytes() / (1024.0 * 1024.0));
-    if (ta.dt > dt_max)
-        throw std::runtime_error("time step violates the CFL condition -- reduce dt, reduce the FD order, "
-                                 "or coarsen f0/dx");
-    if (ppw < 3.0f)
-        std::fprintf(stderr, "WARNING: fewer than 3 points per wavelength; the image will be visibly "
-                             "dispersive.\n");
-
-    // ---- migration -------------------------------------------------------
-    std::vector<float> image, illum;
-    std::printf("=== MIGRATION (%s) ===\n", engine->name());
-    engine->migrate(shots, image, illum);
-
-    // ---- outputs: the raw zero-lag cross-correlation is the reference image
-    io::write_raw_floats(out_path, image);
-    std::printf("\nwrote %s  (%d x %d float32, image[ix*nz+iz])\n", out_path.c_str(), g.nx, g.nz);
-    const auto dot = out_path.find_last_of('.');
-    const std::string stem = dot == std::string::npos ? out_path : out_path.substr(0, dot);
-    if (want_illum_map) io::write_raw_floats(stem + "_illummap.bin", illum);
-    // post-processed copies (src/main.cpp:211-222): on the device, from the
-    // image the HIP engine keeps resident after migrate
-    auto* hip = dynamic_cast<HipRTM*>(engine.get());
-    if (want_illum) {
-        std::vector<float> tmp;
-        if (hip) {
-            hip->postprocess(RTM_HIP_POST_ILLUM, 1e-4f, tmp);
-        } else {
-            tmp = image;
-            illumination_compensation(tmp, illum, 1e-4f);
-        }
-        io::write_raw_floats(stem + "_illum.bin", tmp);
-        std::printf("wrote %s_illum.bin\n", stem.c_str());
-    }
-    if (filter == "laplacian") {
-        std::vector<float> tmp;
-        if (hip) {
-            hip->postprocess(RTM_HIP_POST_LAPLACIAN | (want_illum ? RTM_HIP_POST_ILLUM : 0), 1e-4f, tmp);
-        } else {
-            tmp = image;
-            if (want_illum) illumination_compensation(tmp, illum, 1e-4f);
-            laplacian_filter(tmp, g);
-        }
-        io::write_raw_floats(stem + "_filtered.bin", tmp);
-        std::printf("wrote %s_filtered.bin\n", stem.c_str());
-    }
-
-    // ---- benchmark report -------------------------------------------------
-    BenchmarkContext ctx;
-    ctx.nx = g.nx; ctx.nz = g.nz; ctx.nb = g.nb; ctx.dx = g.dx; ctx.dz = g.dz;
-    ctx.nt = ta.nt; ctx.dt = ta.dt; ctx.nshots = (int)shots.size(); ctx.order = par.order;
-    ctx.store_interval = par.store_interval;
-    ctx.snapshot_bytes = engine->snapshot_bytes();
-    ctx.results.push_back({engine->name(), engine->times, engine->is_gpu()});
-    print_benchmark_report(std::cout, ctx);
-    if (!bench_path.empty()) {
-        std::ofstream bf(bench_path, std::ios::app);
-        print_benchmark_report(bf, ctx);
-    }
-    if (!bench_csv_path.empty()) {
-        RunInfo info;
-        info.host = host_name();
-        info.cpu_name = cpu_model_name();
-        info.gpu_name = engine->device_name();
-        info.engine = engine_name;
-        info.dataset = dataset_label.empty() ? file_stem(shots_path) : dataset_label;
-        info.threads = engine->num_threads();
-        info.gpus = engine->num_devices();
-        append_benchmark_csv(bench_csv_path, ctx, ctx.results.back(), info);
-        std::printf("appended benchmark row to %s\n", bench_csv_path.c_str());
-    }
-    return 0;
-} catch (const std::exception& e) {
-    std::fprintf(stderr, "\nERROR: %s\n", e.what());
-    return 1;
 }
