@@ -13,6 +13,13 @@ __host__ __device__ inline float* rotate(const JobDesc& J, int r) {
     return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
 }
 
+// The field written by backward step s of job J: the rotation's buffer, or
+// on a Hold step of paired imaging a hold buffer (pair_step).
+__host__ __device__ inline float* bwd_out(const JobDesc& J, int s) {
+    const PairStep p = pair_step(J.nt - 1 - s, J.store, J.pair_first);
+    return p.role == PairRole::Hold ? J.hold[p.hold] : rotate(J, (J.bwd_base + s + 2) % 3);
+}
+
 // Step arguments of job J after `total` of its steps (forward steps
 // 0..nt-1, the bookkeeping tick of a 16-bit job, then backward steps
 // 0..nt-1); written to the tick-argument table by k_tick_args, or built on
@@ -29,9 +36,15 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     // other field (the flags serve the other mappings)
     a.zero_prev = s <= 1;
     a.zero_cur = s == 0;
-    a.p_prev = a.zero_prev ? J.zero : rotate(J, (base + s) % 3);
-    a.p_cur = a.zero_cur ? J.zero : rotate(J, (base + s + 1) % 3);
-    a.p_next = rotate(J, (base + s + 2) % 3);
+    if (fwd) {
+        a.p_prev = a.zero_prev ? J.zero : rotate(J, (base + s) % 3);
+        a.p_cur = a.zero_cur ? J.zero : rotate(J, (base + s + 1) % 3);
+        a.p_next = rotate(J, (base + s + 2) % 3);
+    } else {
+        a.p_prev = a.zero_prev ? J.zero : bwd_out(J, s - 2);
+        a.p_cur = a.zero_cur ? J.zero : bwd_out(J, s - 1);
+        a.p_next = bwd_out(J, s);
+    }
     a.coef = J.coef;
     a.vdt2 = J.vdt2;
     a.wx = J.wx;
@@ -63,8 +76,17 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     // the last forward field, imaged at backward step 0 (its slices were
     // reduced by the bookkeeping tick)
     const bool pending = !wide && !fwd && stored && s == 0;
+    // backward imaging by pairing role: none on a Hold step; a Pair step also
+    // images the held stored step, slot + 1
+    const PairRole role = fwd ? PairRole::None : pair_step(it, st, J.pair_first).role;
+    const bool imaging = !fwd && stored && role != PairRole::Hold;
+    const bool paired = role == PairRole::Pair;
+    a.hold_field = paired ? J.hold[pair_step(it, st, J.pair_first).hold] : nullptr;
+    a.snap_in2 = (wide && paired) ? J.snap32 + (slot + 1) * J.n_int : nullptr;
+    a.snap16_in2 = (!wide && paired) ? J.snap16 + (slot + 1) * J.n_int : nullptr;
+    a.snap16_max2 = (!wide && paired) ? J.snap_max + slot + 1 : nullptr;
     a.snap_out = (wide && fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
-    a.snap_in = (wide && !fwd && stored) ? J.snap32 + slot * J.n_int : nullptr;
+    a.snap_in = (wide && imaging) ? J.snap32 + slot * J.n_int : nullptr;
     a.wave_max = (half16 && stored) ? J.wave_max + (ptrdiff_t)p0 * J.parts_cap : nullptr;
     a.wave_count = (half16 && stored) ? J.part_counts + p0 : nullptr;
     a.red_in = red ? J.wave_max + (ptrdiff_t)p1 * J.parts_cap : nullptr;
@@ -76,9 +98,9 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     a.quant_out = quant ? J.snap16 + (it - 2) / st * J.n_int : nullptr;
     a.quant_max_out = quant ? J.snap_max + (it - 2) / st : nullptr;
     a.parts_cap = J.parts_cap;
-    a.snap16_max = (!wide && !fwd && stored && !pending) ? J.snap_max + slot : nullptr;
+    a.snap16_max = (!wide && imaging && !pending) ? J.snap_max + slot : nullptr;
     a.pending_field = pending ? rotate(J, J.bwd_base) : nullptr;
-    a.snap16_in = (!wide && !fwd && stored && !pending) ? J.snap16 + slot * J.n_int : nullptr;
+    a.snap16_in = (!wide && imaging && !pending) ? J.snap16 + slot * J.n_int : nullptr;
     return a;
 }
 

@@ -304,7 +304,7 @@ size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
     const size_t snap_elem = m.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
     const size_t tiles_bound = (size_t)(m.nxe / 2 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 2, strip >= 52
     size_t bytes = 0;
-    bytes += 3 * m.n_fld * sizeof(float);                   // three wavefields
+    bytes += 5 * m.n_fld * sizeof(float);                   // three wavefields, two held for paired imaging
     bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store
     if (snap_elem == 2)                                     // spare word, scales and their bookkeeping
         bytes += ((size_t)m.nsnap + 2 * tiles_bound + 2 * rtm_hip::kReduceWaves + 5) * sizeof(float);
@@ -444,6 +444,9 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     d.bwd_base = (m.nt + 1) % 3;            // buffer of the forward phase's last field
     d.vdt2 = m.vdt2; d.wx = m.wx; d.wz = m.wz; d.wavelet = m.wavelet;
     d.zero = m.zero;
+    // 16-bit jobs image their top stored step from its pending field at
+    // backward step 0, so their pairing starts one stored step lower
+    d.pair_first = rtm_hip::pair_first(m.nt, m.params.store_interval, d.snap_fmt != RTM_HIP_SNAP_F32);
     float *tr = nullptr, *img = nullptr, *ill = nullptr;
     int rc = job_alloc(b, j.allocs, &tr, j.traces.size(), "traces");
     float* amps = nullptr;
@@ -451,6 +454,7 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     if (!rc) rc = job_alloc(b, j.allocs, &img, m.n_int, "image");
     if (!rc) rc = job_alloc(b, j.allocs, &ill, m.n_int, "illumination");
     for (int k = 0; k < 3 && !rc; ++k) rc = job_alloc(b, j.allocs, &d.fld[k], m.n_fld, "wavefield");
+    for (int k = 0; k < 2 && !rc; ++k) rc = job_alloc(b, j.allocs, &d.hold[k], m.n_fld, "held wavefield");
     if (d.snap_fmt == RTM_HIP_SNAP_F32) {
         if (!rc) rc = job_alloc(b, j.allocs, &d.snap32, (size_t)m.nsnap * m.n_int, "snapshot store");
     } else {
@@ -467,6 +471,9 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     if (e == hipSuccess) e = hipMemset(img, 0, m.n_int * sizeof(float));
     if (e == hipSuccess) e = hipMemset(ill, 0, m.n_int * sizeof(float));
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(d.fld[k], 0, m.n_fld * sizeof(float));
+    // a step writes output cells only: the outer ring and the guard columns
+    // of a hold buffer are read as p^n / p^{n-1} and must stay zero
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemset(d.hold[k], 0, m.n_fld * sizeof(float));
     if (e == hipSuccess && d.snap_max) e = hipMemset(d.snap_max, 0, (size_t)m.nsnap * sizeof(float));
     if (e == hipSuccess && d.part_counts) e = hipMemset(d.part_counts, 0, 4 * sizeof(int));
     if (e != hipSuccess) return b->hip_fail(e, "job upload");

@@ -37,6 +37,8 @@ struct JobDesc {
     float* amps;                 // receiver amplitudes in backward-entry order (StepArgs::amps)
     int amp_count;
     float* fld[3];
+    float* hold[2];              // fields held for paired imaging (pair_step)
+    int pair_first;              // pair_first() of the backward phase
     const float* zero;           // a field of zeros (the model's), read as p^{n-1} / p^n on a phase's first steps
     float* snap32;               // FP32 snapshots, or
     uint16_t* snap16;            // 16-bit codes +

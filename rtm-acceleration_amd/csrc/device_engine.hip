@@ -66,6 +66,7 @@ struct rtm_hip_engine {
     float* d_wz = nullptr;
     float* d_wavelet = nullptr;
     float* d_fld[3] = {nullptr, nullptr, nullptr};
+    float* d_hold[2] = {nullptr, nullptr};   // fields held for paired imaging (pair_step)
     float* d_snap = nullptr;         // FP32 snapshots
     uint16_t* d_snap16 = nullptr;    // FP16/BF16 snapshot codes
     float* d_snap_max = nullptr;     // per-snapshot max|p| (16-bit formats)
@@ -167,6 +168,7 @@ struct rtm_hip_engine {
         };
         f(d_vdt2); f(d_wx); f(d_wz); f(d_wavelet);
         for (auto& p : d_fld) f(p);
+        for (auto& p : d_hold) f(p);
         f(d_snap); f(d_snap16); f(d_snap_max); f(d_wave_max); f(d_slice_max); f(d_part_counts); f(d_image); f(d_illum); f(d_traces); f(d_amps); f(d_rec);
         f(d_stack_i); f(d_stack_s); f(d_post);
         post_cap = 0;
@@ -348,9 +350,29 @@ void set_snapshot_args(const rtm_hip_engine* h, Dir dir, int it, rtm_hip::StepAr
     a.snap16_in = nullptr;
     a.snap16_max = nullptr;
     a.pending_field = nullptr;
+    a.hold_field = nullptr;
+    a.snap_in2 = nullptr;
+    a.snap16_in2 = nullptr;
+    a.snap16_max2 = nullptr;
     const int st = h->store;
     const bool stored = (it % st) == 0;
     const size_t slot = (size_t)(it / st);
+    if (dir == Dir::Backward && stored) {
+        // paired imaging: a Hold step images nothing (its field goes to a
+        // hold buffer, run_loop), a Pair step images the held stored step
+        // (slot + 1) first, then its own
+        const rtm_hip::PairStep p = rtm_hip::pair_step(it, st, rtm_hip::pair_first(h->nt, st, false));
+        if (p.role == rtm_hip::PairRole::Hold) return;
+        if (p.role == rtm_hip::PairRole::Pair) {
+            a.hold_field = h->d_hold[p.hold];
+            if (h->snap_fmt == RTM_HIP_SNAP_F32) {
+                a.snap_in2 = h->d_snap + (slot + 1) * h->n_int;
+            } else {
+                a.snap16_in2 = h->d_snap16 + (slot + 1) * h->n_int;
+                a.snap16_max2 = h->d_snap_max + slot + 1;
+            }
+        }
+    }
     if (h->snap_fmt == RTM_HIP_SNAP_F32) {
         if (!stored) return;
         if (dir == Dir::Forward) a.snap_out = h->d_snap + slot * h->n_int;
@@ -433,12 +455,27 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         (void)hipEventRecord(phase_ev.start, h->stream);
     }
     int ia = 0, ib = 1, ic = 2;   // p_prev, p_cur, p_next
+    // The field written by backward step s: the rotation's buffer (s + 2) % 3,
+    // or on a Hold step of paired imaging a hold buffer, which the following
+    // steps read as p^n, p^{n-1} and the Pair step as the held field.  (The
+    // buffers start zeroed, so steps 0 and 1 read zeros as before.)
+    const int first = rtm_hip::pair_first(h->nt, h->store, false);
+    auto bwd_out = [&](int s) -> float* {
+        const rtm_hip::PairStep p = rtm_hip::pair_step(h->nt - 1 - s, h->store, first);
+        return (snapshots && p.role == rtm_hip::PairRole::Hold) ? h->d_hold[p.hold] : h->d_fld[(s + 2) % 3];
+    };
     for (int step = 0; step < h->nt; ++step) {
         const int it = dir == Dir::Forward ? step : h->nt - 1 - step;
         a.it = it;
-        a.p_prev = h->d_fld[ia];
-        a.p_cur = h->d_fld[ib];
-        a.p_next = h->d_fld[ic];
+        if (dir == Dir::Forward) {
+            a.p_prev = h->d_fld[ia];
+            a.p_cur = h->d_fld[ib];
+            a.p_next = h->d_fld[ic];
+        } else {
+            a.p_prev = step >= 2 ? bwd_out(step - 2) : h->d_fld[step % 3];
+            a.p_cur = step >= 1 ? bwd_out(step - 1) : h->d_fld[1];
+            a.p_next = bwd_out(step);
+        }
         if (snapshots) set_snapshot_args(h, dir, it, a);
 
         const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
@@ -799,7 +836,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
 
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-        const size_t need = h->snap_bytes + 5 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float) +
+        const size_t need = h->snap_bytes + 7 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float) +
                             (h->image_stack ? 2 * h->n_int * sizeof(double) : 0);
         if (need > free_b)
             return h->fail(RTM_HIP_ERR_MEMORY,
@@ -814,6 +851,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (!rc) rc = h->alloc(&h->d_wz, (size_t)h->ldz, "wz");
     if (!rc) rc = h->alloc(&h->d_wavelet, (size_t)nt, "wavelet");
     for (int b = 0; b < 3 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");
+    for (int b = 0; b < 2 && !rc; ++b) rc = h->alloc(&h->d_hold[b], h->n_fld, "held wavefield");
     if (h->snap_fmt == RTM_HIP_SNAP_F32) {
         if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
     } else {
@@ -859,6 +897,9 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (e == hipSuccess && h->d_stack_i) e = hipMemset(h->d_stack_i, 0, h->n_int * sizeof(double));
     if (e == hipSuccess && h->d_stack_s) e = hipMemset(h->d_stack_s, 0, h->n_int * sizeof(double));
     for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMemset(h->d_fld[b], 0, h->n_fld * sizeof(float));
+    // a step writes output cells only: the outer ring and the guard columns
+    // of a hold buffer are read as p^n / p^{n-1} and must stay zero
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemset(h->d_hold[b], 0, h->n_fld * sizeof(float));
     if (e != hipSuccess) {
         h->release();
         return h->hip_fail(e, "table upload");

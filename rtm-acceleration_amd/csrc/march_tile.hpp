@@ -220,6 +220,7 @@ struct SnapCtx {
     bool store_max;    // forward stored step, 16-bit snapshot (per-wave max)
     bool quantise;     // forward, two steps after a stored step (16-bit)
     int k_in;          // backward decode exponent
+    int k_in2;         // paired imaging: the held stored step's decode exponent
     int k_q;           // forward encode exponent
     float q_max;       // the quantised snapshot's max
     float part;        // quant_parts[lane], loaded before the tile's loads
@@ -251,7 +252,7 @@ __device__ __forceinline__ SnapCtx snap_ctx(const StepArgs& a, Dir d) {
         c.store32 = d == Dir::Forward && a.snap_out != nullptr;
         c.store_max = c.quantise = false;
     }
-    c.k_in = c.k_q = 0;
+    c.k_in = c.k_in2 = c.k_q = 0;
     c.q_max = c.part = 0.0f;
     return c;
 }
@@ -273,6 +274,7 @@ __device__ __forceinline__ void snap_scales(const StepArgs& a, SnapCtx& c, int l
         if (c.imaging)
             c.k_in = snapshot_scale_exponent(a.pending_field ? parts_max_of(c.part, a.quant_count, lane)
                                                              : sld(a.snap16_max));
+        if (c.imaging && a.hold_field) c.k_in2 = snapshot_scale_exponent(sld(a.snap16_max2));
         if (c.quantise) {
             c.q_max = parts_max_of(c.part, a.quant_count, lane);
             c.k_q = snapshot_scale_exponent(c.q_max);
@@ -360,7 +362,28 @@ __device__ __forceinline__ float laplacian(const Coeffs& cf, const float* q, int
 struct ColumnOps {
     float pv, vv, wx, iv, sv;
     uint32_t fb;                     // snapshot bits (snapshot_bits)
+    uint32_t fb2;                    // paired imaging: the held stored step's snapshot bits
+    float hv;                        // paired imaging: its field, kept in the hold buffer
 };
+
+// The image and illumination of one cell on an imaging step: o.iv / o.sv
+// plus, on a paired step, the held stored step's f2 b2 / f2^2 first, then this
+// step's f b / f^2 (the reference's order, src/rtm/imaging.cpp:25-40, applied
+// in descending time).
+template <bool S16>
+__device__ __forceinline__ void image_cell(const StepArgs& a, int k_in2, const ColumnOps& o, float fv, float out,
+                                           float& img, float& ill) {
+    float iI = o.iv, sS = o.sv;
+    if (a.hold_field) {                                       // wave-uniform
+        float f2;
+        if constexpr (S16) f2 = decode_code(o.fb2, k_in2, a.snap_fmt);
+        else f2 = __uint_as_float(o.fb2);
+        iI = iI + f2 * o.hv;
+        sS = sS + f2 * f2;
+    }
+    img = iI + fv * out;
+    ill = sS + fv * fv;
+}
 
 template <int H, Dir D, bool S16>
 __device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx& sc, int x, int z, bool z_out,
@@ -371,8 +394,8 @@ __device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx
     o.pv = (ok && !a.zero_prev) ? gbl(a.p_prev)[i] : 0.0f;
     o.vv = ok ? gbl(a.vdt2)[i] : 0.0f;
     o.wx = x < a.nxe ? gbl(a.wx)[x] : 0.0f;
-    o.iv = o.sv = 0.0f;
-    o.fb = 0;
+    o.iv = o.sv = o.hv = 0.0f;
+    o.fb = o.fb2 = 0;
     if constexpr (D == Dir::Backward) {
         const int jx = x - a.nb;
         const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
@@ -380,6 +403,10 @@ __device__ __forceinline__ ColumnOps column_ops(const StepArgs& a, const SnapCtx
         o.fb = in ? snapshot_bits<S16>(a, j, i) : 0u;
         o.iv = in ? gbl(a.image)[j] : 0.0f;
         o.sv = in ? gbl(a.illum)[j] : 0.0f;
+        const bool pair = in && a.hold_field;
+        if constexpr (S16) o.fb2 = pair ? (uint32_t)gbl(a.snap16_in2)[j] : 0u;
+        else o.fb2 = pair ? __float_as_uint(__builtin_nontemporal_load(gbl(a.snap_in2) + j)) : 0u;
+        o.hv = pair ? gbl(a.hold_field)[i] : 0.0f;
     }
     return o;
 }
@@ -428,9 +455,10 @@ __device__ __forceinline__ void store_cell(const StepArgs& a, const SnapCtx& sc,
             }
         } else {
             if (sc.imaging && !a.dry) {
-                const float fv = snapshot_value<S16>(a, sc, o.fb, j);
-                gbl(a.image)[j] = o.iv + fv * out;
-                gbl(a.illum)[j] = o.sv + fv * fv;
+                float img, ill;
+                image_cell<S16>(a, sc.k_in2, o, snapshot_value<S16>(a, sc, o.fb, j), out, img, ill);
+                gbl(a.image)[j] = img;
+                gbl(a.illum)[j] = ill;
             }
         }
     }
@@ -642,6 +670,15 @@ __device__ __forceinline__ void strip_imaging_ops(const StepArgs& a, const Strip
         o[r].iv = interior_col(a.image, a, jx)[s.jzc];
         o[r].sv = interior_col(a.illum, a, jx)[s.jzc];
     }
+    if (a.hold_field) {                                       // wave-uniform: a paired step
+#pragma unroll
+        for (int r = 0; r < TX; ++r) {
+            const int jx = s.x0 + r - a.nb;
+            if constexpr (S16) o[r].fb2 = __builtin_nontemporal_load(interior_col(a.snap16_in2, a, jx) + s.jzc);
+            else o[r].fb2 = __float_as_uint(__builtin_nontemporal_load(interior_col(a.snap_in2, a, jx) + s.jzc));
+            o[r].hv = field_col(a.hold_field, a, s, s.x0 + r)[s.lz];
+        }
+    }
 }
 
 // p^{n+1} of output cell (x, lane) to the field, and its snapshot / imaging
@@ -665,9 +702,10 @@ __device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc
         }
     } else {
         if (sc.imaging && !a.dry) {
-            const float fv = strip_snapshot<S16>(a, sc, o.fb);
-            gbl(a.image)[j] = o.iv + fv * out;
-            gbl(a.illum)[j] = o.sv + fv * fv;
+            float img, ill;
+            image_cell<S16>(a, sc.k_in2, o, strip_snapshot<S16>(a, sc, o.fb), out, img, ill);
+            gbl(a.image)[j] = img;
+            gbl(a.illum)[j] = ill;
         }
     }
 }
@@ -707,8 +745,8 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         ops[r].pv = pp[(ptrdiff_t)r * a.ldz + s.lz];
         ops[r].vv = pv2[(ptrdiff_t)r * a.ldz + s.lz];
         ops[r].wx = sld(a.wx + x0 + r);
-        ops[r].iv = ops[r].sv = 0.0f;
-        ops[r].fb = 0;
+        ops[r].iv = ops[r].sv = ops[r].hv = 0.0f;
+        ops[r].fb = ops[r].fb2 = 0;
     }
     if constexpr (D == Dir::Backward && !LEAN)
         if (sc.imaging) strip_imaging_ops<TX, S16>(a, s, ops);

@@ -103,6 +103,15 @@ struct StepArgs {
     const float*    snap_in;
     const uint16_t* snap16_in;
     const float*    snap16_max;
+    // paired imaging (pair_step): this step images two stored steps in one
+    // pass over the image, first the earlier one in backward order, whose
+    // field was kept in hold_field and whose snapshot is snap_in2 (FP32) or
+    // snap16_in2 with max *snap16_max2, then its own; hold_field is null on
+    // every other step
+    const float*    hold_field;
+    const float*    snap_in2;
+    const uint16_t* snap16_in2;
+    const float*    snap16_max2;
     float*       image;
     float*       illum;
     // batched phases: the first steps of a phase read p_prev/p_cur as zero
@@ -116,6 +125,46 @@ struct StepArgs {
     const float* pending_field;
     int          dry;
 };
+
+// Paired imaging of the backward phase.  Every stored step (it % store == 0)
+// images I += f b, S += f^2 with its snapshot f and its field b, in
+// descending it.  Imaging two stored steps in one pass reads and writes I and
+// S once instead of twice: from the stored step `first` down, consecutive
+// stored steps pair up; the first of a pair (Hold) writes its field into hold
+// buffer `hold` and images nothing, the second (Pair), store steps later,
+// images the held field with the earlier snapshot and then its own, in the
+// reference's order, so the sums are unchanged.  A stored step above `first`
+// (a batched job's pending 16-bit snapshot) or a last one without a partner
+// images alone; first < 0 turns pairing off.  Two hold buffers alternate by
+// pair, so a Hold never overwrites a field still read (store = 1 and 2
+// included).
+enum class PairRole : int { None = 0, Alone = 1, Hold = 2, Pair = 3 };
+struct PairStep {
+    PairRole role;
+    int hold;      // hold buffer written (Hold) or read (Pair)
+};
+__host__ __device__ inline PairStep pair_step(int it, int store, int first) {
+    PairStep p{PairRole::None, 0};
+    if (it % store) return p;
+    p.role = PairRole::Alone;
+    if (first < 0 || it > first) return p;
+    const int c = (first - it) / store;              // stored steps since `first`
+    if (c & 1) {
+        p.role = PairRole::Pair;
+        p.hold = ((c - 1) >> 1) & 1;
+    } else if (it - store >= 0) {
+        p.role = PairRole::Hold;
+        p.hold = (c >> 1) & 1;
+    }
+    return p;
+}
+// The first stored step of the backward phase that can be held: the top
+// stored step, or the one below it when the top one is imaged from its
+// still-pending field (pending_top).
+__host__ __device__ inline int pair_first(int nt, int store, bool pending_top) {
+    const int top = (nt - 1) / store * store;
+    return pending_top && top == nt - 1 ? top - store : top;
+}
 
 // Whether a step has no snapshot work, so a lean build of the step (one
 // without the snapshot epilogue) can run it: a backward step that does not

@@ -42,8 +42,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     const int lx0 = wave * RX;               // first local column of this wave
 
     // ---- the streamed operands first, so their latency overlaps the staging
-    float pv[RX], vv[RX], wxv[RX], iv[RX], sv[RX];
-    uint32_t fb[RX];
+    float pv[RX], vv[RX], wxv[RX];
+    ColumnOps im[RX];                       // imaging operands (fb, fb2, hv, iv, sv)
 #pragma unroll
     for (int r = 0; r < RX; ++r) {
         const int ix = x0 + lx0 + r;
@@ -55,9 +55,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
         const int jx = ix - a.nb;
         const bool in = sc.imaging && ok && z_int && jx >= 0 && jx < a.nx;
         const ptrdiff_t j = (ptrdiff_t)jx * a.nz + jz;
-        fb[r] = in ? snapshot_bits<S16>(a, j, i) : 0u;
-        iv[r] = in ? a.image[j] : 0.0f;
-        sv[r] = in ? a.illum[j] : 0.0f;
+        im[r].fb = in ? snapshot_bits<S16>(a, j, i) : 0u;
+        im[r].iv = in ? a.image[j] : 0.0f;
+        im[r].sv = in ? a.illum[j] : 0.0f;
+        const bool pair = in && a.hold_field;
+        if constexpr (S16) im[r].fb2 = pair ? (uint32_t)a.snap16_in2[j] : 0u;
+        else im[r].fb2 = pair ? __float_as_uint(__builtin_nontemporal_load(a.snap_in2 + j)) : 0u;
+        im[r].hv = pair ? a.hold_field[i] : 0.0f;
     }
 
     // ---- stage p^n (tile + halo) in LDS; out-of-grid cells read as 0 -------
@@ -126,9 +130,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
                 }
             } else {
                 if (sc.imaging && !a.dry) {
-                    const float fv = snapshot_value<S16>(a, sc, fb[r], j);
-                    a.image[j] = iv[r] + fv * out;
-                    a.illum[j] = sv[r] + fv * fv;
+                    float img, ill;
+                    image_cell<S16>(a, sc.k_in2, im[r], snapshot_value<S16>(a, sc, im[r].fb, j), out, img, ill);
+                    a.image[j] = img;
+                    a.illum[j] = ill;
                 }
             }
         }

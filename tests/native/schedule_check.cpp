@@ -35,7 +35,11 @@ int main(int argc, char** argv) {
         J.n_int = kInt;
         J.parts_cap = kCap;
         J.bwd_base = (nt + 1) % 3;
+        // fields 0-2, hold buffers 3-4, the zero field 5 (fld() below)
         for (int k = 0; k < 3; ++k) J.fld[k] = (float*)(uintptr_t)(0x1000000u * (k + 1));
+        for (int k = 0; k < 2; ++k) J.hold[k] = (float*)(uintptr_t)(0x1000000u * (k + 4));
+        J.zero = (const float*)(uintptr_t)(0x1000000u * 6);
+        J.pair_first = rtm_hip::pair_first(nt, store, fmt != 0);
         J.snap32 = (float*)(uintptr_t)0x10000000u;
         J.snap16 = (uint16_t*)(uintptr_t)0x20000000u;
         J.wave_max = (float*)(uintptr_t)0x30000000u;
@@ -56,7 +60,8 @@ int main(int argc, char** argv) {
                 "\"zp\":%d,\"zc\":%d,\"snap_out\":%lld,\"snap_in\":%lld,\"wmax\":%lld,\"wcount\":%lld,"
                 "\"red_in\":%lld,\"red_count\":%lld,\"red_out\":%lld,\"red_out_count\":%lld,"
                 "\"qparts\":%lld,\"qcount\":%lld,\"qout\":%lld,\"qmax\":%lld,"
-                "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld,\"lean\":%d}\n",
+                "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld,\"lean\":%d,"
+                "\"hold\":%lld,\"snap_in2\":%lld,\"s16in2\":%lld,\"s16max2\":%lld}\n",
                 argv[c], total, total < nt ? "fwd" : "bwd", a.it, fld(a.p_prev), fld(a.p_cur), fld(a.p_next),
                 a.zero_prev, a.zero_cur, idx(a.snap_out, 0x10000000u, kInt * 4), idx(a.snap_in, 0x10000000u, kInt * 4),
                 idx(a.wave_max, 0x30000000u, kCap * 4), idx(a.wave_count, 0x50000000u, 4),
@@ -65,7 +70,9 @@ int main(int argc, char** argv) {
                 idx(a.quant_parts, 0x40000000u, rtm_hip::kReduceWaves * 4), idx(a.quant_count, 0x50000000u, 4),
                 idx(a.quant_out, 0x20000000u, kInt * 2), idx(a.quant_max_out, 0x60000000u, 4),
                 idx(a.snap16_in, 0x20000000u, kInt * 2), idx(a.snap16_max, 0x60000000u, 4), fld(a.pending_field),
-                (int)rtm_hip::step_is_lean(total < nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, a, s16));
+                (int)rtm_hip::step_is_lean(total < nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, a, s16),
+                fld(a.hold_field), idx(a.snap_in2, 0x10000000u, kInt * 4), idx(a.snap16_in2, 0x20000000u, kInt * 2),
+                idx(a.snap16_max2, 0x60000000u, 4));
         }
     }
     return 0;

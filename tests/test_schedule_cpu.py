@@ -4,8 +4,11 @@ checked on the host by following the data through it rather than by
 restating its formulas:
 
 * every step reads, as p^n and p^{n-1}, the buffers that the two previous
-  steps of its phase wrote, reads zero exactly where the phase has no such
-  step, and its three buffers are distinct;
+  steps of its phase wrote, reads the zero field exactly where the phase has
+  no such step, and writes a buffer it does not read;
+* paired imaging: every snapshot slot is imaged exactly once, in descending
+  order; a paired step images the held slot first, and the held buffer then
+  still holds the field of that slot's backward step (no step overwrote it);
 * FP32: every snapshot slot is written once in the forward phase by the
   step that produces that field and read once in the backward phase at the
   matching time;
@@ -63,15 +66,24 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
     assert len(stored) == nsnap
 
     # ---- fields: what each buffer holds, followed through both phases
-    content = {0: None, 1: None, 2: None}
+    # (buffers 0-2 rotate, 3-4 are the hold buffers, 5 is the zero field)
+    ZERO = 5
+    content = {k: None for k in range(5)}
     for phase, recs in (("F", fwd), ("B", bwd)):
         for s, r in enumerate(recs):
-            assert len({r["prev"], r["cur"], r["next"]}) == 3
             assert r["zc"] == (s == 0) and r["zp"] == (s <= 1)
+            assert (r["cur"] == ZERO) == (s == 0) and (r["prev"] == ZERO) == (s <= 1)
+            assert r["next"] not in (ZERO, r["prev"], r["cur"], r["hold"])
             if s >= 1:
                 assert content[r["cur"]] == (phase, s - 1)
             if s >= 2:
                 assert content[r["prev"]] == (phase, s - 2)
+            if r["hold"] >= 0:
+                # the held field is that of the backward step of the held slot
+                k2 = r["snap_in2"] if not s16 else r["s16in2"]
+                assert phase == "B" and k2 >= 0 and content[r["hold"]] == ("B", nt - 1 - k2 * store)
+            else:
+                assert r["snap_in2"] == r["s16in2"] == r["s16max2"] == -1
             if phase == "B" and s == 0 and r["pending"] >= 0:
                 # the last forward field, imaged straight from its buffer
                 assert content[r["pending"]] == ("F", nt - 1)
@@ -87,12 +99,30 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
             assert r["lean"] == (r["s16in"] == -1 and r["pending"] == -1)
         else:
             assert r["lean"] == (r["snap_out"] == -1 and r["snap_in"] == -1)
-        if r["kind"] == "bwd" or not s16:
+        if r["kind"] == "fwd" and not s16:
             assert r["lean"] == (r["it"] not in stored)
+
+    # backward imaging, in the order the kernels apply it: the held slot of a
+    # paired step first, then the step's own
+    order = []
+    for r in bwd:
+        own, held = (r["s16in"], r["s16in2"]) if s16 else (r["snap_in"], r["snap_in2"])
+        if s16 and r["pending"] >= 0:
+            own = r["it"] // store
+        if held >= 0:
+            assert own == held - 1
+            order.append(held)
+        if own >= 0:
+            assert r["it"] == own * store
+            order.append(own)
+    assert order == list(range(nsnap - 1, -1, -1))
+    # pairing pays: at most one stored step images alone besides a pending one
+    alone = [r for r in bwd if r["it"] in stored and r["hold"] < 0 and
+             (r["snap_in"] >= 0 or r["s16in"] >= 0) and r["pending"] < 0]
+    assert len(alone) <= 1
 
     if not s16:
         assert [r["snap_out"] for r in fwd] == [it // store if it in stored else -1 for it in range(nt)]
-        assert [r["snap_in"] for r in bwd] == [r["it"] // store if r["it"] in stored else -1 for r in bwd]
         assert all(r["wmax"] == r["red_in"] == r["qout"] == r["s16in"] == r["pending"] == -1 for r in fwd + bwd)
         return
 
@@ -126,17 +156,13 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
     # step nt-1), and slot nt-1's slices for backward step 0
     if nt >= 2 and (nt - 2) % store == 0:
         assert any(v == ("F", nt - 2) for v in last_fwd.values())
-    # backward: each stored slot imaged once, the last forward field pending
-    imaged = {}
+    # backward: the last forward field is imaged pending, at step 0; every
+    # other imaged slot reads its codes and its scale (the held one too)
     for s, r in enumerate(bwd):
         it = r["it"]
-        if it in stored:
+        if it in stored and s == 0:
             k = it // store
-            if s == 0:
-                assert r["pending"] >= 0 and r["s16in"] == -1 and r["qparts"] == (k & 1) and r["qcount"] == 2 + (k & 1)
-            else:
-                assert r["pending"] == -1 and r["s16in"] == k and r["s16max"] == k
-            imaged[k] = imaged.get(k, 0) + 1
+            assert r["pending"] >= 0 and r["s16in"] == -1 and r["qparts"] == (k & 1) and r["qcount"] == 2 + (k & 1)
         else:
-            assert r["s16in"] == -1 and r["pending"] == -1
-    assert imaged == {k: 1 for k in range(nsnap)}
+            assert r["pending"] == -1
+            assert r["s16max"] == r["s16in"] and r["s16max2"] == r["s16in2"]
