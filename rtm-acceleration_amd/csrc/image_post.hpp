@@ -21,15 +21,17 @@ namespace rtm_hip {
 // Partial maxima written by launch_image_post's first kernel.
 constexpr int kPostMaxParts = 512;
 
-// Scratch the launches need, in floats: the partial maxima, the rounded FP64
-// stacks (when the source is the FP64 stack) and one intermediate image.
-inline size_t post_scratch_floats(size_t n) { return kPostMaxParts + 3 * n; }
+// Scratch the launches need, in floats, for an image of n cells: the partial
+// maxima only, whatever n (the FP64 stacks are rounded where they are read,
+// and the filter of the compensated image compensates the values it reads).
+inline size_t post_scratch_floats(size_t /*n*/) { return kPostMaxParts; }
 
 // flags: bit 0 = I / (S + eps_relative*max(S) + 1e-30f), bit 1 = Laplacian
 // high-pass (applied after bit 0 when both are set).  The source is either
 // the FP32 image and illumination (image, illum) or the FP64 stacks
 // (stack_i, stack_s), rounded once to float first, as the stack download
-// does.  out receives nx*nz floats; the sources are left unchanged.
+// does.  scratch holds post_scratch_floats(nx*nz) floats; out receives nx*nz
+// floats; the sources are left unchanged.
 hipError_t launch_image_post(const float* image, const float* illum, const double* stack_i,
                              const double* stack_s, int nx, int nz, float idx2, float idz2,
                              int flags, float eps_relative, float* scratch, float* out,
