@@ -178,6 +178,16 @@ int         rtm_hip_image_get_f64(rtm_hip_engine* h, double* image, double* illu
 #define RTM_HIP_POST_LAPLACIAN  2
 int         rtm_hip_image_postprocess(rtm_hip_engine* h, int flags, float eps_relative, float* out);
 
+/* rtm_hip_image_postprocess with no allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and bits; d_out is a device buffer of nx*nz floats the caller
+ * owns; the only scratch (512 partial maxima, 2 KiB) is a static device array
+ * of the library, one per GPU, shared by every call on that GPU: calls that
+ * overlap on one GPU race.  Calls no hipMalloc, hipFree or allocating routine;
+ * returns when the result is in d_out. */
+int         rtm_hip_image_postprocess_noalloc(rtm_hip_engine* h, int flags, float eps_relative, float* d_out);
+
 /* Replaces RTMEngine::migrate (include/rtm_engine.hpp:45-47; src/rtm/rtm.cpp:83-103;
  * src/cuda/rtm_cuda.cu:214-250): zero the image, forward + backward every shot
  * in order (FP32 running sum on the device), download image and illum. */

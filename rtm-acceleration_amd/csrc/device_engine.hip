@@ -31,6 +31,17 @@
 using rtm_hip::Dir;
 using rtm_hip::Entry;
 
+// Scratch of rtm_hip_image_postprocess_noalloc: the partial maxima of S, the
+// only device memory the post-processing kernels use beyond their inputs and
+// output.  kNoAllocScratchFloats floats (2 KiB) per GPU, held until the
+// process ends; every call on a GPU shares that GPU's copy (calls that
+// overlap on one GPU race) and writes every partial it reads, so nothing is
+// carried over from the previous call.  Host code uses its device address
+// (rtm_hip_engine::d_scratch, resolved once per handle in rtm_hip_create),
+// never this name.
+constexpr int kNoAllocScratchFloats = rtm_hip::kPostMaxParts;
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
 namespace {
 
 thread_local std::string g_null_handle_error;
@@ -84,6 +95,7 @@ struct rtm_hip_engine {
     double* d_stack_i = nullptr;
     double* d_stack_s = nullptr;
     float* d_post = nullptr;         // post-processing scratch + output (allocated on first use)
+    float* d_scratch = nullptr;      // this device's copy of `scratch` (rtm_hip_image_postprocess_noalloc)
     size_t post_cap = 0;
     int* d_tile_ptr[2] = {nullptr, nullptr};
     size_t tile_ptr_cap[2] = {0, 0};
@@ -763,6 +775,14 @@ int rtm_hip_create(int device, rtm_hip_engine** out) {
         delete h;
         return RTM_HIP_ERR_DEVICE;
     }
+    // this device's copy of the no-allocation scratch (device is current)
+    e = hipGetSymbolAddress(reinterpret_cast<void**>(&h->d_scratch), HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !h->d_scratch) {
+        g_null_handle_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return RTM_HIP_ERR_DEVICE;
+    }
     *out = h;
     return RTM_HIP_OK;
 }
@@ -1008,9 +1028,9 @@ int rtm_hip_image_postprocess(rtm_hip_engine* h, int flags, float eps_relative, 
                                         "RTM_HIP_POST_LAPLACIAN", flags);
     if (h->nx > 65535) return h->fail(RTM_HIP_ERR_ARG, "post-processing supports nx <= 65535 (nx = %d)", h->nx);
     (void)hipSetDevice(h->device);
-    const size_t scratch = rtm_hip::post_scratch_floats(h->n_int);
-    if (int rc = h->grow(&h->d_post, &h->post_cap, scratch + h->n_int, "post-processing buffer")) return rc;
-    float* d_out = h->d_post + scratch;
+    const size_t parts = rtm_hip::post_scratch_floats(h->n_int);
+    if (int rc = h->grow(&h->d_post, &h->post_cap, parts + h->n_int, "post-processing buffer")) return rc;
+    float* d_out = h->d_post + parts;
     // the reference forms the inverse squared spacings on the host in float
     // (src/rtm/imaging.cpp:65-66)
     const float idx2 = 1.0f / (h->dx * h->dx);
@@ -1022,6 +1042,33 @@ int rtm_hip_image_postprocess(rtm_hip_engine* h, int flags, float eps_relative, 
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(out, d_out, h->n_int * sizeof(float), hipMemcpyDeviceToHost);
     return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image post-processing");
+}
+
+// The same post-processing with no allocation: the partial maxima go to the
+// file-scope scratch of this device, the result to the caller's device
+// buffer; no hipMalloc, hipFree or host copy.
+int rtm_hip_image_postprocess_noalloc(rtm_hip_engine* h, int flags, float eps_relative, float* d_out) {
+    if (int rc = check_handle(h, true)) return rc;
+    if (!d_out) return h->fail(RTM_HIP_ERR_ARG, "null post-processing output");
+    if (flags <= 0 || (flags & ~(RTM_HIP_POST_ILLUM | RTM_HIP_POST_LAPLACIAN)))
+        return h->fail(RTM_HIP_ERR_ARG, "post-processing flags %d: expected RTM_HIP_POST_ILLUM and/or "
+                                        "RTM_HIP_POST_LAPLACIAN", flags);
+    if (h->nx > 65535) return h->fail(RTM_HIP_ERR_ARG, "post-processing supports nx <= 65535 (nx = %d)", h->nx);
+    const size_t need = rtm_hip::post_scratch_floats(h->n_int);
+    if (need > (size_t)kNoAllocScratchFloats)
+        return h->fail(RTM_HIP_ERR_ARG, "post-processing of %zu cells needs %zu scratch floats, more than the %d of "
+                                        "the no-allocation scratch", h->n_int, need, kNoAllocScratchFloats);
+    if (!h->d_scratch) return h->fail(RTM_HIP_ERR_DEVICE, "no-allocation scratch is not resolved on this device");
+    hipError_t e = hipSetDevice(h->device);
+    const float idx2 = 1.0f / (h->dx * h->dx);
+    const float idz2 = 1.0f / (h->dz * h->dz);
+    if (e == hipSuccess)
+        e = rtm_hip::launch_image_post(
+            h->image_stack ? nullptr : h->d_image, h->image_stack ? nullptr : h->d_illum,
+            h->image_stack ? h->d_stack_i : nullptr, h->image_stack ? h->d_stack_s : nullptr, h->nx, h->nz, idx2,
+            idz2, flags, eps_relative, h->d_scratch, d_out, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? RTM_HIP_OK : h->hip_fail(e, "image post-processing (no allocation)");
 }
 
 int rtm_hip_migrate(rtm_hip_engine* h, int nshots, const rtm_hip_shot* shots, float* image, float* illum) {

@@ -99,6 +99,7 @@ SIGNATURES = {
     "rtm_hip_image_get": (C.c_int, [_H, _f32p, _f32p]),
     "rtm_hip_image_get_f64": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "rtm_hip_image_postprocess": (C.c_int, [_H, C.c_int, C.c_float, _f32p]),
+    "rtm_hip_image_postprocess_noalloc": (C.c_int, [_H, C.c_int, C.c_float, C.c_void_p]),
     "rtm_hip_migrate": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _f32p, _f32p]),
     "rtm_hip_model_shot": (C.c_int, [_H, C.POINTER(Shot), _f32p]),
     "rtm_hip_get_times": (C.c_int, [_H, C.POINTER(Times)]),
