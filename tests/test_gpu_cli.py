@@ -92,13 +92,14 @@ def test_segy_input_cli(tmp_path):
     assert sha(out) == man["sha256"]["image.f32"]
 
 
-@pytest.mark.parametrize("engine,label,tol", [("hip-fp16", "FP16 snapshots", 1e-3),
-                                              ("hip-bf16", "BF16 snapshots", 1e-2)])
+@pytest.mark.parametrize("engine,label,tol", [("hip-fp16", "FP16 snapshots", 2e-4),
+                                              ("hip-bf16", "BF16 snapshots", 1.5e-3)])
 def test_16bit_engines_cli(tmp_path, engine, label, tol):
     """rtm --engine hip-fp16 / hip-bf16 (16-bit snapshots with a per-snapshot
-    scale, FP64 stack) on the layered workload: the image stays within the
-    16-bit tolerance of the reference's FP32 image (relative L2; 1e-3 for FP16
-    with its 11-bit significand, 1e-2 for BF16 with 8 bits)."""
+    scale, FP64 stack) on the layered workload: the image stays within about
+    three times the measured 16-bit error of the reference's FP32 image
+    (relative L2 6.6e-5 for FP16 with its 11-bit significand, 4.3e-4 for BF16
+    with 8 bits), so an encoding or scale regression of a few times fails."""
     d = os.path.join(GOLD, "layered3")
     vel = os.path.join(d, "velocity.bin")
     shots = tmp_path / "shots.bin"
@@ -114,3 +115,30 @@ def test_16bit_engines_cli(tmp_path, engine, label, tol):
     rel = np.linalg.norm(img - ref) / np.linalg.norm(ref)
     print(f"{engine}: relative L2 to the reference FP32 image {rel:.3g}", flush=True)
     assert 0 < rel <= tol
+
+
+def test_unstable_dt_refused(tmp_path):
+    """A time step beyond the CFL limit is refused before any propagation
+    (src/main.cpp:188-190): exit status 1, the reason on stderr, the ratio
+    flagged in the summary; a stable one on the same model runs."""
+    nx, nz, dx = 61, 41, 10.0
+    vel = tmp_path / "v.bin"
+    np.full(nx * nz, 3000.0, np.float32).tofile(vel)
+    (tmp_path / "v.bin.hdr").write_text(f"nx={nx}\nnz={nz}\ndx={dx}\ndz={dx}\n")
+    rx = np.arange(0, nx * dx, 20.0, dtype=np.float32)
+    for dt, ok in ((0.0025, False), (0.001, True)):
+        from types import SimpleNamespace
+        nt = 60
+        shot = SimpleNamespace(sx=300.0, sz=20.0, rx=rx, rz=np.full_like(rx, 20.0), nt=nt, dt=dt, nrec=rx.size)
+        shots = tmp_path / f"s{dt}.bin"
+        import golden_data as G
+        shots.write_bytes(G.rtms_bytes([shot], [np.zeros((rx.size, nt), np.float32)]))
+        print(f"case: cli cfl dt={dt}", flush=True)
+        r = subprocess.run([os.path.join(BIN, "rtm"), "--velocity", str(vel), "--shots", str(shots), "--output",
+                            str(tmp_path / "i.bin"), "--order", "8", "--nb", "20"], capture_output=True, text=True,
+                           timeout=300)
+        if ok:
+            assert r.returncode == 0, r.stderr[-1000:]
+        else:
+            assert r.returncode == 1 and "CFL" in r.stderr and "UNSTABLE" in r.stdout, (r.stdout[-800:], r.stderr)
+            assert not (tmp_path / "i.bin").exists()
