@@ -576,9 +576,9 @@ int rebuild_tables(rtm_hip_batch* b) {
 // Whether no job of group `key` has snapshot work `ahead` ticks after the
 // host's clocks (step_is_lean on the job's step arguments, as job_args sets
 // them up on the device), so the tick can run the group's lean build: no
-// FP32 job stores or images, no 16-bit job is in its forward phase or
-// images.  Jobs done by then idle, and a 16-bit job's bookkeeping tick does
-// not run the march tile.  Other kernel families have no lean build.
+// job stores, reduces or quantises a snapshot, and none images.  Jobs done
+// by then idle, and a 16-bit job's bookkeeping tick does not run the march
+// tile.  Other kernel families have no lean build.
 bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead) {
     if (g.family != rtm_hip::Family::kMarchDpp) return false;
     const bool s16 = key_snap16(key);

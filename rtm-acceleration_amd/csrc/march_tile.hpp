@@ -714,9 +714,9 @@ __device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc
 // March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
-// LEAN: a step with no snapshot work (a backward step that does not image,
-// an FP32 forward step that stores no snapshot), built without that
-// epilogue: a smaller kernel for three steps in four.
+// LEAN: a step with no snapshot work (step_is_lean: a backward step that
+// does not image, a forward step with no snapshot store, maxima, reduction
+// or codes), built without that epilogue.
 // t must be wave-uniform (a value read-first-lane'd or derived from one).
 template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
 __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
@@ -724,7 +724,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     const Strip s = make_strip<H>(a, t, lane, TX);
     const int x0 = s.x0;
     SnapCtx sc = snap_ctx<S16>(a, D);
-    if constexpr (LEAN) sc.imaging = sc.store32 = false;
+    if constexpr (LEAN) sc.imaging = sc.store32 = sc.store_max = sc.quantise = false;
     snap_prefetch<S16>(a, sc, lane);
     // The entry bounds are scalar loads: issued before the tile's loads they
     // are in flight with them; the 16-bit build has its slice-max load there

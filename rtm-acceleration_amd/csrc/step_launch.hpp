@@ -203,17 +203,15 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
         }
     } else if (step_is_lean(D, a, S16)) {
         // the DPP march family has lean builds for steps without snapshot work
-        if constexpr (D == Dir::Backward || !S16) {
-            switch (g.tile_x) {
-                case 2:  RTM_LAUNCH((k_march<H, 2, D, true, S16, true>)); break;
-                case 3:  RTM_LAUNCH((k_march<H, 3, D, true, S16, true>)); break;
-                case 4:  RTM_LAUNCH((k_march<H, 4, D, true, S16, true>)); break;
-                case 5:  RTM_LAUNCH((k_march<H, 5, D, true, S16, true>)); break;
-                case 6:  RTM_LAUNCH((k_march<H, 6, D, true, S16, true>)); break;
-                case 8:  RTM_LAUNCH((k_march<H, 8, D, true, S16, true>)); break;
-                case 16: RTM_LAUNCH((k_march<H, 16, D, true, S16, true>)); break;
-                default: return hipErrorInvalidValue;
-            }
+        switch (g.tile_x) {
+            case 2:  RTM_LAUNCH((k_march<H, 2, D, true, S16, true>)); break;
+            case 3:  RTM_LAUNCH((k_march<H, 3, D, true, S16, true>)); break;
+            case 4:  RTM_LAUNCH((k_march<H, 4, D, true, S16, true>)); break;
+            case 5:  RTM_LAUNCH((k_march<H, 5, D, true, S16, true>)); break;
+            case 6:  RTM_LAUNCH((k_march<H, 6, D, true, S16, true>)); break;
+            case 8:  RTM_LAUNCH((k_march<H, 8, D, true, S16, true>)); break;
+            case 16: RTM_LAUNCH((k_march<H, 16, D, true, S16, true>)); break;
+            default: return hipErrorInvalidValue;
         }
     } else {
         switch (g.tile_x) {

@@ -26,6 +26,19 @@ def test_rtm_lists_engines_and_requires_inputs():
     assert r.returncode == 1 and "unknown option: --bogus" in r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(REPO, "oracle", "_ref", "rtm_hip")),
+                    reason="oracle/_ref/rtm_hip is built only where the reference sources are present")
+def test_reference_driver_lists_hip():
+    """The reference's own rtm (src/main.cpp), built with the hip engines
+    registered in its factory (oracle/Makefile), lists them after its own."""
+    r = subprocess.run([os.path.join(REPO, "oracle", "_ref", "rtm_hip"), "--list-engines"], capture_output=True,
+                       text=True)
+    assert r.returncode == 0 and r.stdout.split() == ["cpu", "cpu-opt", "hip", "hip-fp16", "hip-bf16"]
+    r = subprocess.run([os.path.join(REPO, "oracle", "_ref", "rtm_hip"), "--engine", "hip2"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "--velocity, --shots and --output are required" in r.stderr
+
+
 def test_rtm_compare_gate(tmp_path):
     ref = os.path.join(GOLD, "image.f32")
     assert run("rtm_compare", ref, ref, "--nx", "201", "--nz", "151").returncode == 0
