@@ -26,6 +26,17 @@ Reported (one JSON line):
                runs summarised in profiles/step_kernel_traffic_2301x751.md
                (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading
                for the copy probe of known size), when that file is present
+  batched      the batch engine on 8 shots of the same workload at once
+  crops        64 aperture crops of the model (widths U[400,1200], nt
+               U[2000,8000], orders 8 and 12) submitted at once to the batch
+               engine, against the plain engine migrating them one by one;
+               both timings include engine creation, allocation and setup
+  served       seeded Poisson streams (8/s and 64/s, 16 requests each) over
+               three resident models served in real time by the batch engine,
+               against one plain engine serving them first come first served:
+               throughput, p50 and p99 latency and their ratios
+  whole_shot_hbm_frac  algorithmic bytes of whole shots / elapsed time / 8 TB/s,
+               for every line (fields, tables, snapshot store and load, image)
 
 --nt shortens the shot for counter-collection runs only; the reported line
 always uses the full nt = 8000 unless --nt is given.
@@ -112,13 +123,25 @@ def cpu_baseline(model, params, dt, shot, budget_s: float = 20.0):
     wall = time.perf_counter() - t0
     cells = 2.0 * nt_sample * nxe * nze
     out = {"value": round(cells / wall * 1e-9, 4), "unit": "Gcell-steps/s", "cores": int(threads),
-           "kind": kind,
+           "kind": kind, "cpu_model": cpu_model(),
            "sample": f"cpu-opt engine, 1 shot of the 2301x751 workload, nt={nt_sample} of {shot.nt} "
                      f"({cells * 1e-9:.2f} G cell-steps, {wall:.1f} s wall, {prop:.1f} s propagation), "
                      f"IEEE subnormals kept, {os.cpu_count()} host cpus visible"}
     if kind == "reference":
         out["cpu_engine"] = cpu_single_thread(model, params, dt, shot, budget_s / 2)
     return out
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as the reference's CSV
+    records it (src/benchmark/benchmark.cpp cpu_model_name)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_single_thread(model, params, dt, shot, budget_s: float):
@@ -172,20 +195,24 @@ def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=
             "image_checksum": float(np.abs(img).sum())}
 
 
-def crops_workload(args):
-    """64 aperture crops (widths U[400,1200], nt U[2000,8000], orders 8/12)
-    batched, against the plain engine migrating the same crops one by one."""
+def crops_run(ncrops: int, use_graph: bool = True) -> dict:
+    """ncrops aperture crops (widths U[400,1200], nt U[2000,8000], orders
+    8/12) migrated by the batch engine, all submitted at once, and by the
+    plain engine one after the other.  Both timings include everything a
+    caller pays: engine creation, device allocation, model upload and setup,
+    step-kernel choice, the migration and the image download."""
     import rtm_amd
     from rtm_amd import serve
-    specs, reqs = serve.crop_batch(args.crops)
+    specs, reqs = serve.crop_batch(ncrops)
     print(f"bench: modelling {len(reqs)} crop shots", file=sys.stderr, flush=True)
     for sp in specs.values():
         with rtm_amd.HipEngine(0) as e:
             e.setup(sp.model, sp.params, sp.nt, sp.dt)
             sp.shots[0].traces = e.model_shot(sp.shots[0])
     cells = sum(sp.cell_steps for sp in specs.values())
-    print("bench: per-shot loop", file=sys.stderr, flush=True)
-    plain_t = []
+    hbm_bytes = sum(algorithmic_bytes(sp.model, sp.params, sp.nt)[1] for sp in specs.values())
+    print("bench: crops one by one on the plain engine", file=sys.stderr, flush=True)
+    plain_t, migrate_t = [], []
     t_all = time.perf_counter()
     for k in sorted(specs):
         sp = specs[k]
@@ -193,68 +220,103 @@ def crops_workload(args):
         with rtm_amd.HipEngine(0) as e:
             e.setup(sp.model, sp.params, sp.nt, sp.dt)
             e.set_step_kernel(rtm_amd.STEP_MARCH_DPP, 4)    # no per-crop autotuning
+            t_m = time.perf_counter()
             e.migrate([sp.shots[0]])
+            migrate_t.append(time.perf_counter() - t_m)
         plain_t.append(time.perf_counter() - t)
     plain_elapsed = time.perf_counter() - t_all
-    print("bench: batched", file=sys.stderr, flush=True)
-    with rtm_amd.BatchEngine(0, use_graph=not args.batch_no_graph) as b:
-        t_res = time.perf_counter()
-        b.reserve()                               # one allocation for all jobs, before the timed region
-        reserve_s = time.perf_counter() - t_res
-        t0 = time.perf_counter()
+    print("bench: crops batched", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+        b.reserve()                               # one block for every job
         ids = [b.submit(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
                for sp in (specs[k] for k in sorted(specs))]
         b.run()
-        elapsed = time.perf_counter() - t0
         lat = [b.state(j)[1] for j in ids]
+        for j in ids:
+            b.result(j)                           # the download retires the job
+        elapsed = time.perf_counter() - t0
         st = b.stats()
-    value = cells / elapsed * 1e-9
     done = np.cumsum(plain_t)                     # all submitted at t=0, served in order
     return {
-        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded faulted-layer model, aperture crops; traces modelled on the GPU)",
-        "config": {"workload": f"{len(reqs)} aperture crops of the 2301x751 @4 m model, widths U[400,1200], "
-                               "nz 751, nt U[2000,8000], dt 0.35 ms, orders 8/12, nb 60, store 4, FP32 "
-                               "snapshots, all submitted at once to the batch engine", "engine": "hip-batched"},
-        "shots_per_s": round(len(reqs) / elapsed, 4),
+        "crops": len(reqs), "value": round(cells / elapsed * 1e-9, 3), "unit": "Gcell-steps/s",
+        "elapsed_s": round(elapsed, 3), "shots_per_s": round(len(reqs) / elapsed, 4),
+        "whole_shot_hbm_frac": round(hbm_bytes / elapsed * 1e-9 / HBM_PEAK_GBS, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
-        "plain_per_shot_loop": {"timing": "setup + migrate per crop, march-dpp tile 4 step kernel",
+        "timing": "both sides: engine creation, allocation, setup, kernel choice, migration, image download",
+        "plain_per_shot_loop": {"step_kernel": "march-dpp tile 4",
                                 "value": round(cells / plain_elapsed * 1e-9, 3),
+                                "elapsed_s": round(plain_elapsed, 3),
                                 "shots_per_s": round(len(reqs) / plain_elapsed, 4),
+                                "whole_shot_hbm_frac": round(hbm_bytes / plain_elapsed * 1e-9 / HBM_PEAK_GBS, 4),
+                                "migrate_only_s": round(sum(migrate_t), 3),
                                 "latency_ms": {"p50": round(float(np.percentile(done, 50)) * 1e3, 2),
                                                "p99": round(float(np.percentile(done, 99)) * 1e3, 2)}},
         "speedup_vs_plain": round(plain_elapsed / elapsed, 3),
+        # without engine creation, allocation and setup on either side: the
+        # plain engine's migrate calls against the batch engine's run
+        "speedup_vs_plain_migrate_only": round(sum(migrate_t) / st["run_seconds"], 3),
         "batch": {"ticks": st["ticks"], "rebuilds": st["rebuilds"], "groups": st["groups"],
-                  "reserve_s_untimed": round(reserve_s, 3),
                   "seconds": {k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds",
                                                            "retire_seconds")}},
     }
 
 
-def served_workload(args):
-    """Poisson streams over three resident models, swept in rate, against the
-    plain engine serving the same requests serially (FIFO)."""
+def crops_workload(args):
+    """The crops run as its own bench line."""
+    r = crops_run(args.crops, use_graph=not args.batch_no_graph)
+    return {
+        "metric": METRIC, "value": r["value"], "unit": "Gcell-steps/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(r["elapsed_s"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded faulted-layer model, aperture crops; traces modelled on the GPU)",
+        "config": {"workload": f"{r['crops']} aperture crops of the 2301x751 @4 m model, widths U[400,1200], "
+                               "nz 751, nt U[2000,8000], dt 0.35 ms, orders 8/12, nb 60, store 4, FP32 "
+                               "snapshots, all submitted at once to the batch engine", "engine": "hip-batched"},
+        "crops": r,
+    }
+
+
+def served_sweep(rates, jobs: int, reserve_gib: int, chunk_ms: float, use_graph: bool = True,
+                 plain_serial: bool = True) -> list:
+    """Seeded Poisson streams over three resident models at each rate, served
+    in real time by the batch engine, against one plain engine serving the
+    same requests first come first served (its service times measured, the
+    queueing replayed on the same arrival times)."""
     from rtm_amd import serve
     specs = serve.served_models()
     print("bench: modelling traces of the served models", file=sys.stderr, flush=True)
     serve.model_traces(specs)
-    rates = [float(r) for r in args.rates.split(",")]
     sweep = []
     for rate in rates:
-        reqs = serve.served_stream(specs, rate, args.jobs, seed=int(rate * 1000) + 7)
+        reqs = serve.served_stream(specs, rate, jobs, seed=int(rate * 1000) + 7)
         print(f"bench: serving {len(reqs)} requests at {rate}/s", file=sys.stderr, flush=True)
-        batched = serve.serve(specs, reqs, reserve=args.serve_reserve_gib << 30, use_graph=not args.batch_no_graph,
-                              chunk_s=args.serve_chunk_ms * 1e-3)
-        plain = serve.serial_plain(specs, reqs) if not args.no_plain_serial else None
+        batched = serve.serve(specs, reqs, reserve=reserve_gib << 30, use_graph=use_graph, chunk_s=chunk_ms * 1e-3)
+        hbm = sum(algorithmic_bytes(specs[r.key].model, specs[r.key].params, specs[r.key].nt, 2)[1] for r in reqs)
+        batched["whole_shot_hbm_frac"] = hbm / batched["makespan_s"] * 1e-9 / HBM_PEAK_GBS
         row = {"rate_per_s": rate, "batched": _round(batched)}
-        if plain:
+        if plain_serial:
+            plain = serve.serial_plain(specs, reqs)
+            plain["whole_shot_hbm_frac"] = hbm / plain["makespan_s"] * 1e-9 / HBM_PEAK_GBS
             row.update({"plain_serial": _round(plain),
                         "shots_per_s_ratio": round(batched["shots_per_s"] / plain["shots_per_s"], 3),
+                        "p50_ratio": round(batched["latency_p50_s"] / plain["latency_p50_s"], 3),
                         "p99_ratio": round(batched["latency_p99_s"] / plain["latency_p99_s"], 3)})
         sweep.append(row)
+    return sweep
+
+
+SERVED_WORKLOAD = ("served stream: requests uniform over 201x151 @10 m (nt 1400), 1201x401 @8 m (nt 4000), "
+                   "2301x751 @4 m (nt 8000) resident models, order 8, FP32 fields, FP16 snapshots")
+
+
+def served_workload(args):
+    """Poisson streams over three resident models, swept in rate, against the
+    plain engine serving the same requests serially (FIFO)."""
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = served_sweep(rates, args.jobs, args.serve_reserve_gib, args.serve_chunk_ms,
+                         use_graph=not args.batch_no_graph, plain_serial=not args.no_plain_serial)
     top = sweep[-1]
     return {
         "metric": METRIC, "value": round(top["batched"]["gcell_steps_per_s"], 3), "unit": "Gcell-steps/s",
@@ -262,9 +324,7 @@ def served_workload(args):
         "ms_per_step": round(top["batched"]["makespan_s"] / args.jobs * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded models; traces modelled on the GPU; seeded Poisson arrivals)",
-        "config": {"workload": "served stream: requests uniform over 201x151 @10 m (nt 1400), "
-                               "1201x401 @8 m (nt 4000), 2301x751 @4 m (nt 8000) resident models, order 8, "
-                               "FP32 fields, FP16 snapshots; value at the highest rate",
+        "config": {"workload": SERVED_WORKLOAD + "; value at the highest rate",
                    "engine": "hip-batched", "rates": rates, "jobs_per_rate": args.jobs},
         "shots_per_s": round(top["batched"]["shots_per_s"], 4),
         "latency_ms": {"p50": round(top["batched"]["latency_p50_s"] * 1e3, 2),
@@ -336,12 +396,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--family", type=int, default=2,
-                    help="step-kernel family: 2 = DPP march (default; the fastest on this grid in the full "
-                         "loop), -1 = autotune, 0 LDS tile, 1 march, 3 stream")
-    ap.add_argument("--tile", type=int, default=3,
-                    help="step-kernel tile width (default 3: 4 %% faster than 4 in the full loop, within the "
-                         "autotuner's 5 %% margin on its synthetic launches; 0 = built-in)")
+    ap.add_argument("--family", type=int, default=-1,
+                    help="step-kernel family: -1 = the engine's autotuner (default), 0 LDS tile, 1 march, "
+                         "2 DPP march, 3 stream")
+    ap.add_argument("--tile", type=int, default=0, help="step-kernel tile width (0 = the autotuner's)")
     ap.add_argument("--batch-family", type=int, default=0,
                     help="batch step kernel (0 = autotuned, 1 march, 2 DPP march, 3 stream)")
     ap.add_argument("--batch-tile", type=int, default=0, help="batch step kernel tile width (0 = default)")
@@ -363,7 +421,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="plain workload without the batched comparison")
     ap.add_argument("--batched-shots", type=int, default=8,
                     help="shots the batch engine migrates at once in the plain workload's comparison")
-    ap.add_argument("--crops", type=int, default=64, help="crops in the crops workload")
+    ap.add_argument("--crops", type=int, default=64, help="crops in the crops run")
+    ap.add_argument("--no-crops", action="store_true", help="plain workload without the crops run")
+    ap.add_argument("--no-served", action="store_true", help="plain workload without the served sweep")
+    ap.add_argument("--line-rates", default="8,64",
+                    help="arrival rates (1/s) of the served sweep in the plain workload's line")
+    ap.add_argument("--line-jobs", type=int, default=16, help="requests per rate in that sweep")
     ap.add_argument("--rates", default="1,4,16,64", help="arrival rates (1/s) of the served workload")
     ap.add_argument("--jobs", type=int, default=32, help="requests per rate in the served workload")
     ap.add_argument("--no-plain-serial", action="store_true", help="served workload without the plain baseline")
@@ -470,6 +533,13 @@ def main():
                                 tile_x=args.batch_tile)
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
+    if not args.no_crops:
+        line["crops"] = crops_run(args.crops, use_graph=not args.batch_no_graph)
+    if not args.no_served:
+        rates = [float(r) for r in args.line_rates.split(",")]
+        line["served"] = {"workload": SERVED_WORKLOAD, "jobs_per_rate": args.line_jobs,
+                          "sweep": served_sweep(rates, args.line_jobs, args.serve_reserve_gib, args.serve_chunk_ms,
+                                                use_graph=not args.batch_no_graph)}
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(model, params, dt, shots[0])

@@ -17,9 +17,10 @@ Two workloads, both seeded (numpy PCG64) so a run can be repeated exactly:
                           15 Hz, 767 receivers, store 4;
                       all order 8, nb 60, FP32 fields with FP16 snapshots.
 
-serve() drives a BatchEngine in real time: requests are submitted when their
-arrival time passes, the engine runs short chunks of ticks in between, and
-each request's latency runs from its arrival to its image being on the host.
+serve() drives a BatchEngine in real time: requests are admitted after their
+arrival time passes (first come first served, within a fill limit of cells
+per tick), the engine runs short chunks of ticks in between, and each
+request's latency runs from its arrival to its image being on the host.
 serial_plain() times the same requests on the plain per-shot engine, one
 after the other, and derives the FIFO latency a single plain engine would
 give for the same arrival times.
@@ -55,10 +56,19 @@ class ModelSpec:
     shots: list = field(default_factory=list)   # ShotRecords with modelled traces
 
     @property
+    def cells_per_tick(self) -> float:
+        """Extended cells one step of a job on this model updates."""
+        return float((self.model.nx + 2 * self.params.nb) * (self.model.nz + 2 * self.params.nb))
+
+    @property
     def cell_steps(self) -> float:
-        nxe = self.model.nx + 2 * self.params.nb
-        nze = self.model.nz + 2 * self.params.nb
-        return 2.0 * self.nt * nxe * nze
+        return 2.0 * self.nt * self.cells_per_tick
+
+
+# Admission limit of serve(): about the extended grid of one 2301 x 751 job
+# (2421 x 871 cells), the size at which one job's step already spreads over
+# every CU with several waves each.
+FILL_CELLS = 2.2e6
 
 
 def served_models(nsources: int = 4, snapshot_format: int = SNAP_F16) -> dict[str, ModelSpec]:
@@ -106,27 +116,44 @@ def served_stream(specs: dict[str, ModelSpec], rate: float, njobs: int, seed: in
 
 
 def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0, chunk_s: float = 0.004,
-          use_graph: bool = True, memory_budget: int = 0, keep_images: bool = False, reserve: int = 0) -> dict:
+          use_graph: bool = True, memory_budget: int = 0, keep_images: bool = False, reserve: int = 0,
+          fill_cells: float = FILL_CELLS) -> dict:
     """Runs the stream through one BatchEngine in real time; fills each
     request's latency (and image when keep_images).  reserve > 0 places all
     job memory in one block of that many bytes, reserved before the stream
-    starts (-1: the whole budget).  Returns a summary."""
+    starts (-1: the whole budget).
+
+    Admission is first come first served with a fill limit: an arrived
+    request joins the running batch while the batch's extended cells per
+    tick, its own included, stay within fill_cells, or when nothing runs;
+    otherwise it waits, and those behind it too.  One 2301 x 751 job already
+    fills the chip, so large jobs run one after the other (processor sharing
+    among them would delay every one of them to the end of the busy period),
+    while small ones share ticks.  fill_cells <= 0 admits every request on
+    arrival.  Returns a summary."""
     with BatchEngine(device, memory_budget=memory_budget, use_graph=use_graph) as b:
         mids = {k: b.add_model(sp.model, sp.params, sp.nt, sp.dt, sp.snapshot_format) for k, sp in specs.items()}
         if reserve:
             b.reserve(max(reserve, 0))     # the server's job memory, placed before requests arrive
         pending = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
-        out: dict[int, tuple[int, float]] = {}     # job id -> (request index, submit lag)
+        out: dict[int, tuple[int, float, float]] = {}   # job id -> (request index, submit lag, cells per tick)
+        waiting: list[int] = []                          # arrived, not yet admitted, in arrival order
         nxt = 0
         chunk = 16
         t0 = time.perf_counter()
-        while nxt < len(pending) or out:
+        while nxt < len(pending) or out or waiting:
             now = time.perf_counter() - t0
             while nxt < len(pending) and requests[pending[nxt]].arrival <= now:
-                r = requests[pending[nxt]]
-                jid = b.submit_shot(mids[r.key], r.shot)
-                out[jid] = (pending[nxt], (time.perf_counter() - t0) - r.arrival)
+                waiting.append(pending[nxt])
                 nxt += 1
+            while waiting:
+                r = requests[waiting[0]]
+                cells = specs[r.key].cells_per_tick
+                busy = sum(c for _, _, c in out.values())
+                if out and fill_cells > 0 and busy + cells > fill_cells:
+                    break
+                jid = b.submit_shot(mids[r.key], r.shot)
+                out[jid] = (waiting.pop(0), (time.perf_counter() - t0) - r.arrival, cells)
             if not out:
                 time.sleep(max(0.0, requests[pending[nxt]].arrival - (time.perf_counter() - t0)))
                 continue
@@ -139,7 +166,7 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
             for jid in list(out):
                 state, lat = b.state(jid)
                 if state == 2:
-                    i, lag = out.pop(jid)
+                    i, lag, _ = out.pop(jid)
                     r = requests[i]
                     r.latency = lag + lat
                     img, ill = b.result(jid)
@@ -150,6 +177,7 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
     extra = {k: st[k] for k in ("ticks", "rebuilds", "tunes", "groups")}
     extra.update({k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds", "retire_seconds",
                                                   "rebuild_seconds", "capture_seconds")})
+    extra["fill_cells"] = fill_cells
     return summary(requests, makespan, specs, extra=extra)
 
 
