@@ -62,3 +62,22 @@ def test_rtm_without_device_fails_cleanly(tmp_path):
     r = run("rtm", "--velocity", os.path.join(GOLD, "velocity.bin"), "--shots", "x.bin", "--output",
             str(tmp_path / "o.bin"))
     assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def test_bandwidth_probe_csv(tmp_path):
+    """rtm_bandwidth_probe's CSV is the reference's (src/tools/bandwidth_probe.cpp:
+    date,host,device_kind,device_name,threads,gbps, header once, one row per
+    device); the CPU triad row comes first; a bad option exits 1."""
+    csv = tmp_path / "bw.csv"
+    for _ in range(2):
+        r = run("rtm_bandwidth_probe", "--mb", "16", "--repeats", "1", "--csv", str(csv), timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "triad" in r.stdout
+    lines = csv.read_text().splitlines()
+    assert lines[0] == "date,host,device_kind,device_name,threads,gbps"
+    assert lines.count(lines[0]) == 1
+    rows = [ln.split(",") for ln in lines[1:]]
+    assert rows[0][2] == "cpu" and int(rows[0][4]) >= 1 and float(rows[0][5]) > 0.1
+    assert all(len(x) == 6 and x[2] in ("cpu", "gpu") for x in rows)
+    r = run("rtm_bandwidth_probe", "--bogus")
+    assert r.returncode == 1 and "unknown option: --bogus" in r.stderr

@@ -142,3 +142,15 @@ def test_unstable_dt_refused(tmp_path):
         else:
             assert r.returncode == 1 and "CFL" in r.stderr and "UNSTABLE" in r.stdout, (r.stdout[-800:], r.stderr)
             assert not (tmp_path / "i.bin").exists()
+
+
+def test_bandwidth_probe_gpu_row(tmp_path):
+    """On the GPU the probe adds a gpu row: the copy rate of the device, read
+    + write bytes (the reference's rtm_bandwidth_probe GPU half,
+    src/tools/bandwidth_probe_cuda.cu:10-66); on MI355X above 4 TB/s."""
+    csv = tmp_path / "bw.csv"
+    r = run("rtm_bandwidth_probe", "--mb", "256", "--repeats", "3", "--csv", str(csv))
+    print(r.stdout, flush=True)
+    rows = [ln.split(",") for ln in csv.read_text().splitlines()[1:]]
+    assert [x[2] for x in rows] == ["cpu", "gpu"]
+    assert rows[1][4] == "0" and float(rows[1][5]) > 4000.0
