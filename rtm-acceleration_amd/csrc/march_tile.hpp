@@ -689,15 +689,22 @@ __device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc
                                             const ColumnOps& o, float& vmax) {
     field_col(a.p_next, a, s, x)[s.lz] = out;
     const int jx = x - a.nb;
+    if constexpr (D == Dir::Forward && S16) {
+        // one lane predicate for the interior instead of early exits: the
+        // maximum is a select, the code store the only masked instruction
+        const bool in = (unsigned)jx < (unsigned)a.nx && s.z_int;
+        if (sc.store_max) vmax = fmaxf(vmax, in ? fabsf(out) : 0.0f);
+        if (sc.quantise && !a.dry) {
+            const uint16_t code = encode_snapshot(o.pv, sc.k_q, a.snap_fmt);
+            if (in) __builtin_nontemporal_store(code, gbl(a.quant_out) + ((ptrdiff_t)jx * a.nz + s.jzc));
+        }
+        return;
+    }
     if (jx < 0 || jx >= a.nx) return;                       // wave-uniform
     if (!s.z_int) return;
     const ptrdiff_t j = (ptrdiff_t)jx * a.nz + s.jzc;
     if constexpr (D == Dir::Forward) {
-        if constexpr (S16) {
-            if (sc.store_max) vmax = fmaxf(vmax, fabsf(out));
-            if (sc.quantise && !a.dry)
-                __builtin_nontemporal_store(encode_snapshot(o.pv, sc.k_q, a.snap_fmt), gbl(a.quant_out) + j);
-        } else {
+        if constexpr (!S16) {
             if (sc.store32 && !a.dry) __builtin_nontemporal_store(out, gbl(a.snap_out) + j);
         }
     } else {
@@ -714,23 +721,31 @@ __device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc
 // March mapping: one wave64 = one tile of (64 - 2H) x TX outputs, every load
 // of the tile issued before any use.
 // ---------------------------------------------------------------------------
-// LEAN: a step with no snapshot work (step_is_lean: a backward step that
-// does not image, a forward step with no snapshot store, maxima, reduction
-// or codes), built without that epilogue.
+// LEAN: a step with no snapshot work of its own cells (step_is_lean: a
+// backward step that does not image, a forward step with no snapshot store,
+// maxima or codes), built without that epilogue; a 16-bit forward step that
+// only reduces the previous stored step's maxima runs lean.
+// PH (16-bit forward steps): the snapshot work compiled in, 1 the maxima of
+// the step's own field, 2 the codes of the field two steps back, 3 both; a
+// build with only one of them leaves out the other's code.
 // t must be wave-uniform (a value read-first-lane'd or derived from one).
-template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
+template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false, int PH = 3>
 __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, int t, int lane) {
     constexpr int NQ = TX + 2 * H;           // p^n columns a lane needs
+    // a lean step has no snapshot operand of its own: the FP32 build's code
+    constexpr bool S = S16 && !LEAN;
     const Strip s = make_strip<H>(a, t, lane, TX);
     const int x0 = s.x0;
-    SnapCtx sc = snap_ctx<S16>(a, D);
+    SnapCtx sc = snap_ctx<S>(a, D);
     if constexpr (LEAN) sc.imaging = sc.store32 = sc.store_max = sc.quantise = false;
-    snap_prefetch<S16>(a, sc, lane);
+    if constexpr (!(PH & 1)) sc.store_max = false;
+    if constexpr (!(PH & 2)) sc.quantise = false;
+    snap_prefetch<S>(a, sc, lane);
     // The entry bounds are scalar loads: issued before the tile's loads they
     // are in flight with them; the 16-bit build has its slice-max load there
     // already and does better without them.
     int e0 = 0, e1 = 0;
-    if constexpr (!S16) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
+    if constexpr (!S) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
 
     // every load of the tile, issued before any use
     const global_t<const float>* pc = field_col(a.p_cur, a, s, x0 - H);
@@ -749,11 +764,11 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         ops[r].fb = ops[r].fb2 = 0;
     }
     if constexpr (D == Dir::Backward && !LEAN)
-        if (sc.imaging) strip_imaging_ops<TX, S16>(a, s, ops);
+        if (sc.imaging) strip_imaging_ops<TX, S>(a, s, ops);
     const float wzv = gbl(a.wz)[s.z0 + s.lz];
-    if constexpr (S16) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
+    if constexpr (S) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
-    snap_scales<S16>(a, sc, lane);
+    snap_scales<S>(a, sc, lane);
     float vmax = 0.0f;
 
     // One pass over the columns computes every output once.  A tile without
@@ -770,7 +785,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     for (int r = 0; r < TX; ++r) {
         out[r] = settle(leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv));
         const int x = x0 + r;
-        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S16>(a, sc, s, x, out[r], ops[r], vmax);
+        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S>(a, sc, s, x, out[r], ops[r], vmax);
     }
     if (!plain) {
         // a tile with entries waits for its entry registers once, here
@@ -781,9 +796,11 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         apply_tile_entries<D, TX>(a, te, cell, out, ops, e0, e1);
 #pragma unroll
         for (int r = 0; r < TX; ++r)
-            if (cell[r] >= 0) strip_store<D, S16>(a, sc, s, x0 + r, out[r], ops[r], vmax);
+            if (cell[r] >= 0) strip_store<D, S>(a, sc, s, x0 + r, out[r], ops[r], vmax);
     }
-    finish_tile<S16>(a, sc, vmax, lane, t, a.ntiles);
+    finish_tile<S>(a, sc, vmax, lane, t, a.ntiles);
+    // the reduction of the previous stored step's maxima runs in lean
+    // 16-bit forward steps too (it is their only snapshot work)
     settle_scales<S16>(a, sc, t, a.ntiles, lane);
 }
 

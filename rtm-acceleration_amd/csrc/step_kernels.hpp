@@ -167,13 +167,13 @@ __host__ __device__ inline int pair_first(int nt, int store, bool pending_top) {
 }
 
 // Whether a step has no snapshot work, so a lean build of the step (one
-// without the snapshot epilogue) can run it: a backward step that does not
-// image, a forward step that neither stores an FP32 snapshot nor takes part
-// in a 16-bit one (its maxima, their reduction or its codes).  The same
-// tests as the kernels' own (snap_ctx), on the host.
+// without the snapshot epilogue of its cells) can run it: a backward step
+// that does not image, a forward step that neither stores an FP32 snapshot
+// nor the maxima or codes of a 16-bit one (the reduction of the maxima, red_in,
+// runs in the lean build too).  The same tests as the kernels' own
+// (snap_ctx), on the host.
 inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
-    if (d == Dir::Forward)
-        return a.snap_out == nullptr && a.wave_max == nullptr && a.red_in == nullptr && a.quant_out == nullptr;
+    if (d == Dir::Forward) return a.snap_out == nullptr && a.wave_max == nullptr && a.quant_out == nullptr;
     return s16 ? (a.snap16_in == nullptr && a.pending_field == nullptr) : a.snap_in == nullptr;
 }
 

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 }
 
 // March mapping: one wave per tile (march_tile.hpp).
-template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false>
+template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false, int PH = 3>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     // the wave index is wave-uniform: the tile's coordinates and column
     // offsets are then scalar arithmetic
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
     const int t = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
     if (t >= a.ntiles) return;               // whole wave leaves together
-    march_tile<H, TX, D, DPP, S16, LEAN>(a, a.coef, t, lane);
+    march_tile<H, TX, D, DPP, S16, LEAN, PH>(a, a.coef, t, lane);
 }
 
 
@@ -214,16 +214,32 @@ hipError_t launch_h(const StepGeometry& g, const StepArgs& a, hipStream_t s, hip
             default: return hipErrorInvalidValue;
         }
     } else {
-        switch (g.tile_x) {
-            case 2:  RTM_LAUNCH((k_march<H, 2, D, true, S16>)); break;
-            case 3:  RTM_LAUNCH((k_march<H, 3, D, true, S16>)); break;
-            case 4:  RTM_LAUNCH((k_march<H, 4, D, true, S16>)); break;
-            case 5:  RTM_LAUNCH((k_march<H, 5, D, true, S16>)); break;
-            case 6:  RTM_LAUNCH((k_march<H, 6, D, true, S16>)); break;
-            case 8:  RTM_LAUNCH((k_march<H, 8, D, true, S16>)); break;
-            case 16: RTM_LAUNCH((k_march<H, 16, D, true, S16>)); break;
-            default: return hipErrorInvalidValue;
+        // a 16-bit forward step with one of its two kinds of snapshot work
+        // runs the build with only that one
+#define RTM_MARCH_TILES(PHV)                                                        \
+        switch (g.tile_x) {                                                         \
+            case 2:  RTM_LAUNCH((k_march<H, 2, D, true, S16, false, PHV>)); break;  \
+            case 3:  RTM_LAUNCH((k_march<H, 3, D, true, S16, false, PHV>)); break;  \
+            case 4:  RTM_LAUNCH((k_march<H, 4, D, true, S16, false, PHV>)); break;  \
+            case 5:  RTM_LAUNCH((k_march<H, 5, D, true, S16, false, PHV>)); break;  \
+            case 6:  RTM_LAUNCH((k_march<H, 6, D, true, S16, false, PHV>)); break;  \
+            case 8:  RTM_LAUNCH((k_march<H, 8, D, true, S16, false, PHV>)); break;  \
+            case 16: RTM_LAUNCH((k_march<H, 16, D, true, S16, false, PHV>)); break; \
+            default: return hipErrorInvalidValue;                                   \
         }
+        if constexpr (D == Dir::Forward && S16) {
+            const int ph = (a.wave_max != nullptr ? 1 : 0) | (a.quant_out != nullptr ? 2 : 0);
+            if (ph == 1) {
+                RTM_MARCH_TILES(1)
+            } else if (ph == 2) {
+                RTM_MARCH_TILES(2)
+            } else {
+                RTM_MARCH_TILES(3)
+            }
+        } else {
+            RTM_MARCH_TILES(3)
+        }
+#undef RTM_MARCH_TILES
     }
     return hipGetLastError();
 }
