@@ -132,9 +132,18 @@ bool key_snap16(int key) { return (key & 1) != 0; }
 
 }  // namespace
 
+// Streams a run of ticks spreads its launch groups over: groups are
+// independent within a run (each job reads only its own state and the run's
+// tick arguments), so their launches run concurrently and fill each other's
+// tails, where launches on one stream would each drain the device.
+constexpr int kGroupStreams = 3;
+
 struct rtm_hip_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t group_streams[kGroupStreams - 1] = {};   // groups 1, 2, ... of a run (group 0: stream)
+    hipEvent_t fork_ev = nullptr;                       // the run's tick arguments are ready
+    hipEvent_t join_ev[kGroupStreams - 1] = {};         // a group stream's launches of the run are done
     std::string err;
     rtm_hip_batch_config cfg{};
     size_t budget = 0, used = 0;
@@ -612,15 +621,35 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
         return b->fail(RTM_HIP_ERR_ARG, "internal: a run of %d ticks (1..%d)", ticks, rtm_hip::kRunTicks);
     hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, ticks, 0, b->stream);
     if (ea != hipSuccess) return b->hip_fail(ea, "tick arguments");
-    for (int sub = 0; sub < ticks; ++sub) {
-        for (auto& [key, g] : b->groups) {
-            if (g.ntiles == 0) continue;
+    // group i of the run on stream i mod kGroupStreams (0: the handle's own)
+    std::vector<std::pair<int, Group*>> run_groups;
+    for (auto& [key, g] : b->groups)
+        if (g.ntiles) run_groups.push_back({key, &g});
+    const int nstreams = std::min<int>((int)run_groups.size(), kGroupStreams);
+    auto stream_of = [&](size_t i) { return i % kGroupStreams == 0 ? b->stream : b->group_streams[i % kGroupStreams - 1]; };
+    if (nstreams > 1) {
+        hipError_t e = hipEventRecord(b->fork_ev, b->stream);
+        for (int k = 1; k < nstreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->group_streams[k - 1], b->fork_ev, 0);
+        if (e != hipSuccess) return b->hip_fail(e, "group stream fork");
+    }
+    for (size_t i = 0; i < run_groups.size(); ++i) {
+        const int key = run_groups[i].first;
+        const Group& g = *run_groups[i].second;
+        for (int sub = 0; sub < ticks; ++sub) {
             const bool lean = group_tick_lean(b, key, g, ahead + sub);
             hipError_t e = rtm_hip::launch_batch_step(key_half(key), g.family, g.tile_x, key_snap16(key), b->d_jobs,
                                                       b->d_targs, g.d_tile_job, g.ntiles, sub, g.lds_bytes,
-                                                      b->stream, lean);
+                                                      stream_of(i), lean);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
         }
+    }
+    if (nstreams > 1) {
+        hipError_t e = hipSuccess;
+        for (int k = 1; k < nstreams && e == hipSuccess; ++k) {
+            e = hipEventRecord(b->join_ev[k - 1], b->group_streams[k - 1]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join_ev[k - 1], 0);
+        }
+        if (e != hipSuccess) return b->hip_fail(e, "group stream join");
     }
     hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
@@ -1012,6 +1041,15 @@ int rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rtm_hip_ba
         g_batch_null_error = "stream creation failed";
         return RTM_HIP_ERR_DEVICE;
     }
+    bool ok = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < kGroupStreams - 1 && ok; ++k)
+        ok = hipStreamCreateWithFlags(&b->group_streams[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join_ev[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        g_batch_null_error = "stream creation failed";
+        rtm_hip_batch_destroy(b.release());
+        return RTM_HIP_ERR_DEVICE;
+    }
     *out = b.release();
     return RTM_HIP_OK;
 }
@@ -1031,7 +1069,15 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
     if (b->arena.active()) (void)hipFree(b->arena.base());
-    (void)hipStreamDestroy(b->stream);
+    for (int k = 0; k < kGroupStreams - 1; ++k) {
+        if (b->group_streams[k]) {
+            (void)hipStreamSynchronize(b->group_streams[k]);
+            (void)hipStreamDestroy(b->group_streams[k]);
+        }
+        if (b->join_ev[k]) (void)hipEventDestroy(b->join_ev[k]);
+    }
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
