@@ -185,7 +185,8 @@ def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=
                        kernel_family=0, tile_x=0):
     """The batch engine on the same shots: one resident model, every shot
     submitted at once and advanced together, one launch per time step
-    (hipGraph replay).  Returns throughput and per-shot latency."""
+    (direct launches at this tick size; hipGraph replay only for ticks small
+    enough to be launch-bound).  Returns throughput and per-shot latency."""
     import rtm_amd
     with rtm_amd.BatchEngine(0, use_graph=use_graph, kernel_family=kernel_family, tile_x=tile_x) as b:
         b.reserve()                               # job memory placed once, outside the timed region

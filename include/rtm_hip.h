@@ -259,7 +259,8 @@ void        rtm_hip_mute_direct_wave(const rtm_hip_shot* shot, float* traces,
 typedef struct {
     size_t memory_budget;        /* device bytes for jobs; 0 = 90% of free memory    */
     int    max_active;           /* cap on jobs on the device at once; 0 = no cap    */
-    int    use_graph;            /* 1: capture one tick in a hipGraph and replay it  */
+    int    use_graph;            /* 1: replay captured runs of ticks (hipGraph) while ticks
+                                    are small enough that host launches would pace them */
     int    kernel_family;        /* 0: autotuned per stencil order and load; else RTM_HIP_STEP_MARCH,
                                     RTM_HIP_STEP_MARCH_DPP or RTM_HIP_STEP_STREAM      */
     int    tile_x;               /* with kernel_family: its tile width (0: default)  */

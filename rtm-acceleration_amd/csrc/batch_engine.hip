@@ -947,10 +947,25 @@ void count_launches(rtm_hip_batch* b, int ticks, int ahead) {
     b->stats.lean_launches += (long long)std::count(p.begin() + (ptrdiff_t)body, p.end(), '1');
 }
 
+// Whether the host's launches would set the pace of the active set's ticks:
+// a tick of fewer extended cells than about 3M per launch group (~10 us of
+// device time at the step kernels' rate) is over before the host has issued
+// the next tick's launches, and a captured run issues them all at once.
+// Larger ticks run no slower from direct launches, and some faster: 16
+// crops 600 cells wide (10M cells a tick) take 157 ms for 4000 ticks from
+// direct launches against 165 ms from graph replay.
+bool launch_bound(const rtm_hip_batch* b) {
+    double cells = 0.0;
+    for (const Job* j : b->active) cells += (double)j->model->nxe * (double)j->model->nze;
+    int groups = 0;
+    for (const auto& [key, g] : b->groups) groups += g.ntiles > 0;
+    return cells < 3.0e6 * std::max(groups, 1);
+}
+
 int run_ticks(rtm_hip_batch* b, int n) {
     if (b->active.size() == 1) {
         if (int rc = run_solo(b, n)) return rc;
-    } else if (b->cfg.use_graph) {
+    } else if (b->cfg.use_graph && launch_bound(b)) {
         for (int ahead = 0; ahead < n;) {
             const int ticks = n - ahead >= kGraphTicks ? kGraphTicks : 1;
             hipGraphExec_t exec = nullptr;
