@@ -246,7 +246,11 @@ def crops_run(ncrops: int, use_graph: bool = True) -> dict:
     print("bench: crops batched", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
-        b.reserve()                               # one block for every job
+        # one block for every job: their total when it fits the budget (the
+        # device clears what it allocates, so a larger block costs more)
+        need = sum(b.job_bytes(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
+                   for sp in specs.values())
+        b.reserve(int(need * 1.02) + (256 << 20) if need * 1.02 < b.stats()["bytes_budget"] else 0)
         ids = [b.submit(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
                for sp in (specs[k] for k in sorted(specs))]
         b.run()
