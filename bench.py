@@ -213,7 +213,7 @@ def batched_same_shots(model, params, nt, dt, shots, warm_shot, snapshot_format=
             "image_checksum": float(np.abs(img).sum())}
 
 
-def crops_run(ncrops: int, use_graph: bool = True) -> dict:
+def crops_run(ncrops: int, use_graph: bool = True, merge_groups: bool = False) -> dict:
     """ncrops aperture crops (widths U[400,1200], nt U[2000,8000], orders
     8/12) migrated by the batch engine, all submitted at once, and by the
     plain engine one after the other.  Both timings include everything a
@@ -245,7 +245,7 @@ def crops_run(ncrops: int, use_graph: bool = True) -> dict:
     plain_elapsed = time.perf_counter() - t_all
     print("bench: crops batched", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    with rtm_amd.BatchEngine(0, use_graph=use_graph) as b:
+    with rtm_amd.BatchEngine(0, use_graph=use_graph, merge_groups=merge_groups) as b:
         # one block for every job: their total when it fits the budget (the
         # device clears what it allocates, so a larger block costs more)
         need = sum(b.job_bytes(sp.model, sp.params, sp.nt, sp.dt, sp.shots[0], sp.snapshot_format)
@@ -280,6 +280,7 @@ def crops_run(ncrops: int, use_graph: bool = True) -> dict:
         # plain engine's migrate calls against the batch engine's run
         "speedup_vs_plain_migrate_only": round(sum(migrate_t) / st["run_seconds"], 3),
         "batch": {"ticks": st["ticks"], "rebuilds": st["rebuilds"], "groups": st["groups"],
+                  "merged_launch": bool(st["merged_launch"]),
                   "seconds": {k: round(st[k], 4) for k in ("run_seconds", "admit_seconds", "tune_seconds",
                                                            "retire_seconds")}},
     }
@@ -287,7 +288,7 @@ def crops_run(ncrops: int, use_graph: bool = True) -> dict:
 
 def crops_workload(args):
     """The crops run as its own bench line."""
-    r = crops_run(args.crops, use_graph=not args.batch_no_graph)
+    r = crops_run(args.crops, use_graph=not args.batch_no_graph, merge_groups=args.batch_merge)
     return {
         "metric": METRIC, "value": r["value"], "unit": "Gcell-steps/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": round(r["elapsed_s"] * 1e3, 3), "higher_is_better": True,
@@ -427,6 +428,8 @@ def main():
     ap.add_argument("--batch-tile", type=int, default=0, help="batch step kernel tile width (0 = default)")
     ap.add_argument("--batch-snapshots", choices=("f32", "f16", "bf16"), default="f32",
                     help="snapshot format of the batched comparison's jobs")
+    ap.add_argument("--batch-merge", action="store_true",
+                    help="batch engine: one launch per tick for jobs of every stencil order (crops run)")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batched comparison with direct launches instead of hipGraph replay")
     ap.add_argument("--sample-every", type=int, default=-1,
@@ -559,7 +562,7 @@ def main():
         bt["speedup_vs_plain_shots_per_s"] = round(bt["shots_per_s"] / line["shots_per_s"], 3)
         line["batched"] = bt
     if not args.no_crops:
-        line["crops"] = crops_run(args.crops, use_graph=not args.batch_no_graph)
+        line["crops"] = crops_run(args.crops, use_graph=not args.batch_no_graph, merge_groups=args.batch_merge)
     if not args.no_served:
         rates = [float(r) for r in args.line_rates.split(",")]
         line["served"] = {"workload": SERVED_WORKLOAD, "jobs_per_rate": args.line_jobs,

@@ -264,6 +264,9 @@ typedef struct {
     int    kernel_family;        /* 0: autotuned per stencil order and load; else RTM_HIP_STEP_MARCH,
                                     RTM_HIP_STEP_MARCH_DPP or RTM_HIP_STEP_STREAM      */
     int    tile_x;               /* with kernel_family: its tile width (0: default)  */
+    int    merge_groups;         /* 1: one launch per tick for jobs of every stencil order and
+                                    snapshot width (DPP march, 2-4 column tiles); 0: one launch
+                                    per group, the groups on concurrent streams */
 } rtm_hip_batch_config;
 
 /* One shot migration: the arguments of setup (grid, velocity, params, nt, dt)
@@ -306,6 +309,7 @@ typedef struct {
     long long graphs_captured;   /* captured runs of ticks, one per launch pattern  */
     long long step_launches;     /* batched step launches (one per tick and group)  */
     long long lean_launches;     /* of which ran the lean build (no snapshot work)  */
+    int       merged_launch;     /* 1: the groups present run in one launch per tick */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

@@ -157,6 +157,11 @@ struct rtm_hip_batch {
     rtm_hip::TickArgs* d_targs = nullptr;            // [jobs_cap][kRunTicks] (launch_tick_args)
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by group_key (half-width, snapshot width)
+    // One launch per tick for every job (cfg.merge_groups, with jobs of more
+    // than one group and the DPP march family): the tiles of all jobs in one
+    // table, the kernel picking each wave's tile build (launch_batch_merged).
+    Group merged;
+    bool merged_on = false;
     bool dirty = true;
     // captured runs of ticks, by the run's launch pattern: its length and,
     // per tick and group, the lean or full build (a few patterns recur while
@@ -511,6 +516,44 @@ int rebuild(rtm_hip_batch* b) {
     return rc;
 }
 
+// The tile table of one launch over the active jobs idx: the tuning of the
+// launch's load class (when it has been tuned before), every job's entry
+// tables at the launch's tile width, and tile_job uploaded.
+int build_group_table(rtm_hip_batch* b, Group& g, const std::vector<int>& idx) {
+    if (b->cfg.kernel_family == 0) {
+        double cells = 0;
+        for (int k : idx) cells += (double)b->active[k]->model->nxe * b->active[k]->model->nze;
+        const int cls = load_class_of(cells);
+        if (cls != g.load_class) {
+            g.load_class = cls;
+            auto it = g.by_load.find(cls);
+            if (it != g.by_load.end()) static_cast<Tuning&>(g) = it->second;
+            g.tuned = it != g.by_load.end();
+        }
+    }
+    std::vector<int> tile_job;
+    for (int k : idx) {
+        Job& j = *b->active[k];
+        if (j.tile_x_built != g.tile_x)
+            if (int rc = build_job_entries(b, j, g.tile_x)) return rc;
+        j.desc.tile_first = (int)tile_job.size();
+        tile_job.insert(tile_job.end(), (size_t)j.desc.ntiles, k);
+    }
+    g.ntiles = (int)tile_job.size();
+    if ((size_t)g.ntiles > g.cap) {
+        if (g.d_tile_job) (void)hipFree(g.d_tile_job);
+        g.cap = (size_t)g.ntiles;
+        hipError_t e = hipMalloc(&g.d_tile_job, g.cap * sizeof(int));
+        if (e != hipSuccess) {
+            g.d_tile_job = nullptr;
+            g.cap = 0;
+            return b->hip_fail(e, "tile table");
+        }
+    }
+    hipError_t e = hipMemcpy(g.d_tile_job, tile_job.data(), tile_job.size() * sizeof(int), hipMemcpyHostToDevice);
+    return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "tile table upload");
+}
+
 int rebuild_tables(rtm_hip_batch* b) {
     if (hipError_t e = hipStreamSynchronize(b->stream); e != hipSuccess) return b->hip_fail(e, "batch synchronisation");
     b->drop_graph();
@@ -530,42 +573,18 @@ int rebuild_tables(rtm_hip_batch* b) {
     }
     std::map<int, std::vector<int>> by_key;
     for (size_t k = 0; k < b->active.size(); ++k) by_key[group_key(*b->active[k]->model)].push_back((int)k);
+    const int fam = b->cfg.kernel_family;
+    b->merged_on = b->cfg.merge_groups && by_key.size() > 1 && (fam == 0 || fam == RTM_HIP_STEP_MARCH_DPP);
     for (auto& [key, g] : b->groups)
-        if (!by_key.count(key)) g.ntiles = 0;
-    for (auto& [key, idx] : by_key) {
-        Group& g = b->groups[key];
-        if (b->cfg.kernel_family == 0) {
-            double cells = 0;
-            for (int k : idx) cells += (double)b->active[k]->model->nxe * b->active[k]->model->nze;
-            const int cls = load_class_of(cells);
-            if (cls != g.load_class) {
-                g.load_class = cls;
-                auto it = g.by_load.find(cls);
-                if (it != g.by_load.end()) static_cast<Tuning&>(g) = it->second;
-                g.tuned = it != g.by_load.end();
-            }
-        }
-        std::vector<int> tile_job;
-        for (int k : idx) {
-            Job& j = *b->active[k];
-            if (j.tile_x_built != g.tile_x)
-                if (int rc = build_job_entries(b, j, g.tile_x)) return rc;
-            j.desc.tile_first = (int)tile_job.size();
-            tile_job.insert(tile_job.end(), (size_t)j.desc.ntiles, k);
-        }
-        g.ntiles = (int)tile_job.size();
-        if ((size_t)g.ntiles > g.cap) {
-            if (g.d_tile_job) (void)hipFree(g.d_tile_job);
-            g.cap = (size_t)g.ntiles;
-            hipError_t e = hipMalloc(&g.d_tile_job, g.cap * sizeof(int));
-            if (e != hipSuccess) {
-                g.d_tile_job = nullptr;
-                g.cap = 0;
-                return b->hip_fail(e, "tile table");
-            }
-        }
-        hipError_t e = hipMemcpy(g.d_tile_job, tile_job.data(), tile_job.size() * sizeof(int), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return b->hip_fail(e, "tile table upload");
+        if (b->merged_on || !by_key.count(key)) g.ntiles = 0;
+    if (b->merged_on) {
+        std::vector<int> all(b->active.size());
+        for (size_t k = 0; k < all.size(); ++k) all[k] = (int)k;
+        if (int rc = build_group_table(b, b->merged, all)) return rc;
+    } else {
+        b->merged.ntiles = 0;
+        for (auto& [key, idx] : by_key)
+            if (int rc = build_group_table(b, b->groups[key], idx)) return rc;
     }
     // every clock is known on the host: one step per tick since admission
     std::vector<JobDesc> descs;
@@ -588,11 +607,11 @@ int rebuild_tables(rtm_hip_batch* b) {
 // job stores, reduces or quantises a snapshot, and none images.  Jobs done
 // by then idle, and a 16-bit job's bookkeeping tick does not run the march
 // tile.  Other kernel families have no lean build.
-bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead) {
-    if (g.family != rtm_hip::Family::kMarchDpp) return false;
-    const bool s16 = key_snap16(key);
+bool jobs_tick_lean(const rtm_hip_batch* b, int key, int ahead) {
     for (const Job* j : b->active) {
-        if (group_key(*j->model) != key) continue;
+        const int jkey = group_key(*j->model);
+        if (key >= 0 && jkey != key) continue;
+        const bool s16 = key_snap16(jkey);
         const int total = j->desc.steps - j->steps_left + ahead;
         if (total >= j->desc.steps || (s16 && total == j->desc.nt)) continue;
         const rtm_hip::Dir d = total < j->desc.nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward;
@@ -602,14 +621,24 @@ bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead)
     }
     return true;
 }
+bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead) {
+    return g.family == rtm_hip::Family::kMarchDpp && jobs_tick_lean(b, key, ahead);
+}
+// The merged launch: lean when no job at all has snapshot work.
+bool merged_tick_lean(const rtm_hip_batch* b, int ahead) { return jobs_tick_lean(b, -1, ahead); }
 
 // The launch pattern of a run of `ticks` ticks starting `ahead` ticks after
 // the host's clocks: its length, then per tick and group '1' (lean) or '0'.
 std::string run_pattern(const rtm_hip_batch* b, int ticks, int ahead) {
-    std::string p = std::to_string(ticks) + ":";
-    for (int sub = 0; sub < ticks; ++sub)
+    std::string p = std::to_string(ticks) + (b->merged_on ? "m:" : ":");
+    for (int sub = 0; sub < ticks; ++sub) {
+        if (b->merged_on) {
+            p += merged_tick_lean(b, ahead + sub) ? '1' : '0';
+            continue;
+        }
         for (const auto& [key, g] : b->groups)
             if (g.ntiles) p += group_tick_lean(b, key, g, ahead + sub) ? '1' : '0';
+    }
     return p;
 }
 
@@ -621,6 +650,16 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
         return b->fail(RTM_HIP_ERR_ARG, "internal: a run of %d ticks (1..%d)", ticks, rtm_hip::kRunTicks);
     hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, ticks, 0, b->stream);
     if (ea != hipSuccess) return b->hip_fail(ea, "tick arguments");
+    if (b->merged_on) {
+        const Group& g = b->merged;
+        for (int sub = 0; sub < ticks; ++sub) {
+            hipError_t e = rtm_hip::launch_batch_merged(g.tile_x, b->d_jobs, b->d_targs, g.d_tile_job, g.ntiles, sub,
+                                                        b->stream, merged_tick_lean(b, ahead + sub));
+            if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
+        }
+        hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);
+        return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
+    }
     // group i of the run on stream i mod kGroupStreams (0: the handle's own)
     std::vector<std::pair<int, Group*>> run_groups;
     for (auto& [key, g] : b->groups)
@@ -659,6 +698,8 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
 // timing dry steps (no clock advance, no image or snapshot update: the next
 // real step rewrites the same p^{n+1}) on the live jobs; the launches are
 // long, so a candidate replaces the best when it is 2 % faster.
+int autotune_merged(rtm_hip_batch* b);
+
 int autotune(rtm_hip_batch* b) {
     if (b->cfg.kernel_family != 0) {
         for (auto& [key, g] : b->groups) {
@@ -668,8 +709,15 @@ int autotune(rtm_hip_batch* b) {
             g.tuned = true;
             b->dirty = true;
         }
+        if (b->merged_on && !b->merged.tuned) {
+            const std::vector<int> w = rtm_hip::merged_tile_widths();
+            b->merged.tile_x = std::count(w.begin(), w.end(), b->cfg.tile_x) ? b->cfg.tile_x : 4;
+            b->merged.tuned = true;
+            b->dirty = true;
+        }
         return RTM_HIP_OK;
     }
+    if (b->merged_on) return autotune_merged(b);
     bool any = false;
     for (auto& [key, g] : b->groups) any |= g.ntiles > 0 && !g.tuned;
     if (!any) return RTM_HIP_OK;
@@ -767,6 +815,65 @@ int autotune(rtm_hip_batch* b) {
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     return rc;
+}
+
+// The merged launch's tile width (DPP march, merged_tile_widths()), timed on
+// dry ticks of the live jobs as autotune times a group's candidates: widest
+// first, a narrower width must win by 2 %; the choice is kept per load class.
+int autotune_merged(rtm_hip_batch* b) {
+    Group& g = b->merged;
+    if (g.tuned || g.ntiles == 0) return RTM_HIP_OK;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipError_t e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
+    const int rounds = g.by_load.empty() ? 3 : 2, reps = g.by_load.empty() ? 8 : 4;
+    std::vector<int> widths = rtm_hip::merged_tile_widths();
+    std::sort(widths.begin(), widths.end(), [](int x, int y) { return x > y; });
+    float best = 1e30f;
+    int pick = g.tile_x;
+    for (int tx : widths) {
+        if (rc) break;
+        g.tile_x = tx;
+        if ((rc = rebuild(b))) break;
+        if (hipError_t ea = rtm_hip::launch_tick_args(b->d_jobs, (int)b->active.size(), b->d_targs, 1, 1, b->stream);
+            ea != hipSuccess) {
+            rc = b->hip_fail(ea, "tick arguments");
+            break;
+        }
+        float ms = 1e30f;
+        for (int round = 0; round < rounds && !rc; ++round) {
+            for (int rep = 0; rep < (round == 0 ? reps + 2 : reps) && !rc; ++rep) {
+                if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
+                hipError_t le = rtm_hip::launch_batch_merged(g.tile_x, b->d_jobs, b->d_targs, g.d_tile_job, g.ntiles,
+                                                             0, b->stream, false);
+                if (le != hipSuccess) rc = b->hip_fail(le, "autotune launch");
+            }
+            if (rc) break;
+            (void)hipEventRecord(t1, b->stream);
+            if (hipError_t se = hipEventSynchronize(t1); se != hipSuccess) {
+                rc = b->hip_fail(se, "autotune");
+                break;
+            }
+            float round_ms = 0.0f;
+            (void)hipEventElapsedTime(&round_ms, t0, t1);
+            ms = std::min(ms, round_ms);
+        }
+        if (!rc && ms < 0.98f * best) {
+            best = ms;
+            pick = tx;
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    g.tile_x = pick;
+    g.family = rtm_hip::Family::kMarchDpp;
+    g.by_load[g.load_class] = g;
+    g.tuned = true;
+    b->stats.tunes += 1;
+    b->dirty = true;
+    return RTM_HIP_OK;
 }
 
 int admit_from_queue(rtm_hip_batch* b) {
@@ -959,7 +1066,7 @@ bool launch_bound(const rtm_hip_batch* b) {
     for (const Job* j : b->active) cells += (double)j->model->nxe * (double)j->model->nze;
     int groups = 0;
     for (const auto& [key, g] : b->groups) groups += g.ntiles > 0;
-    return cells < 3.0e6 * std::max(groups, 1);
+    return cells < 3.0e6 * std::max(b->merged_on ? 1 : groups, 1);
 }
 
 int run_ticks(rtm_hip_batch* b, int n) {
@@ -1079,6 +1186,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     for (auto& [id, m] : b->models) model_release(b, *m);
     for (auto& [key, g] : b->groups)
         if (g.d_tile_job) (void)hipFree(g.d_tile_job);
+    if (b->merged.d_tile_job) (void)hipFree(b->merged.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_targs) (void)hipFree(b->d_targs);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
@@ -1197,7 +1305,7 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         int warm = 0;
         if (b->cfg.kernel_family == 0)
             for (Job* j : b->active)
-                if (!b->groups[group_key(*j->model)].tuned)
+                if (b->merged_on ? !b->merged.tuned : !b->groups[group_key(*j->model)].tuned)
                     warm = std::max(warm, 2 - (j->desc.steps - j->steps_left));
         if (warm > 0) {
             n = std::min(n, warm);
@@ -1299,14 +1407,17 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
     int k = 0;
     for (auto& [key, g] : b->groups) {
         if (k >= 12) break;
+        // while merged, the groups present run in the one launch, with its tiles
+        const Group& run = b->merged_on ? b->merged : g;
         out->group_half[k] = key_half(key);
         out->group_snap16[k] = key_snap16(key) ? 1 : 0;
-        out->group_tile_x[k] = g.tile_x;
-        out->group_family[k] = (int)g.family;
-        out->group_waves_per_cu[k] = g.blocks_per_cu * rtm_hip::kWaves;
+        out->group_tile_x[k] = run.tile_x;
+        out->group_family[k] = (int)run.family;
+        out->group_waves_per_cu[k] = run.blocks_per_cu * rtm_hip::kWaves;
         ++k;
     }
     out->ngroups = k;
+    out->merged_launch = b->merged_on ? 1 : 0;
     return RTM_HIP_OK;
 }
 

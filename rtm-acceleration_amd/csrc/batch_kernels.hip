@@ -38,6 +38,7 @@ __global__ __launch_bounds__(64) void k_tick_args(const JobDesc* jobs, int njobs
              : (s16 && total == J.nt) ? kTickBookkeeping
                                       : kTickBackward;
     T.tile_first = J.tile_first;
+    T.build = 2 * J.half + (s16 ? 1 : 0);
     if (T.kind == kTickForward || T.kind == kTickBackward) {
         T.a = s16 ? job_args<true>(J, total, dry) : job_args<false>(J, total, dry);
     } else {                                 // bookkeeping reads the descriptor; idle nothing

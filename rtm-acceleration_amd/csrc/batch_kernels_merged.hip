@@ -1,0 +1,87 @@
+// batch_kernels_merged.hip -- one launch per tick for the whole batch.
+//
+// The per-group launches (batch_launch.hpp) run jobs of one stencil
+// half-width and snapshot width each; with several groups a tick is several
+// launches, each with its own ramp and drain.  Here every job's tiles are in
+// one launch: a wave reads its job's half-width with its tick arguments and
+// runs, in a wave-uniform switch, the march tile built for that half-width
+// and snapshot width.  The kernel holds one tile body per (half-width,
+// snapshot width, direction); a wave only ever runs one, and the kernel's
+// registers are the largest body's.
+#include "batch_launch.hpp"
+
+namespace rtm_hip {
+namespace {
+
+using namespace device;
+
+template <int H, int TX, bool S16, bool LEAN>
+__device__ __forceinline__ void merged_tile(const JobDesc& J, const StepArgs& a, int kind, int t, int lane) {
+    if (S16 && kind == kTickBookkeeping) {
+        if (!a.dry) bookkeeping_tile<H, TX>(J, t, lane);
+        return;
+    }
+    if (kind == kTickForward) march_tile<H, TX, Dir::Forward, true, S16, LEAN>(a, a.coef, t, lane);
+    else march_tile<H, TX, Dir::Backward, true, S16, LEAN>(a, a.coef, t, lane);
+}
+
+template <int TX, bool LEAN>
+__global__ __launch_bounds__(kBlockThreads) void k_batch_merged(const JobDesc* __restrict__ jobs,
+                                                                const TickArgs* __restrict__ targs,
+                                                                const int* __restrict__ tile_job, int ntiles, int sub) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const int nblocks = (ntiles + kWaves - 1) / kWaves;
+    const int g = xcd_tile((int)blockIdx.x, nblocks) * kWaves + wave;
+    if (g >= ntiles) return;
+    const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
+    const TickArgs& T = targs[jid * kRunTicks + sub];
+    const int kind = T.kind;
+    if (kind == kTickIdle) return;           // finished job: its tiles idle
+    const int t = g - T.tile_first;
+    const StepArgs& a = T.a;
+    const JobDesc& J = jobs[jid];
+    // the job's (half-width, snapshot width), uniform over the wave; from
+    // the descriptor, since a bookkeeping tick's step arguments are empty
+    switch (__builtin_amdgcn_readfirstlane(T.build)) {
+        case 2:  merged_tile<1, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 3:  merged_tile<1, TX, true, LEAN>(J, a, kind, t, lane); break;
+        case 4:  merged_tile<2, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 5:  merged_tile<2, TX, true, LEAN>(J, a, kind, t, lane); break;
+        case 6:  merged_tile<3, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 7:  merged_tile<3, TX, true, LEAN>(J, a, kind, t, lane); break;
+        case 8:  merged_tile<4, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 9:  merged_tile<4, TX, true, LEAN>(J, a, kind, t, lane); break;
+        case 10: merged_tile<5, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 11: merged_tile<5, TX, true, LEAN>(J, a, kind, t, lane); break;
+        case 12: merged_tile<6, TX, false, LEAN>(J, a, kind, t, lane); break;
+        case 13: merged_tile<6, TX, true, LEAN>(J, a, kind, t, lane); break;
+        default: break;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch_merged(int tile_x, const JobDesc* jobs, const TickArgs* targs, const int* tile_job,
+                               int ntiles, int sub, hipStream_t stream, bool lean) {
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 block(kBlockThreads);
+    const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
+#define RTM_MERGED(TX)                                                                                         \
+    do {                                                                                                       \
+        if (lean) hipLaunchKernelGGL((k_batch_merged<TX, true>), grid, block, 0, stream, jobs, targs, tile_job, \
+                                     ntiles, sub);                                                             \
+        else hipLaunchKernelGGL((k_batch_merged<TX, false>), grid, block, 0, stream, jobs, targs, tile_job,     \
+                                ntiles, sub);                                                                  \
+    } while (0)
+    switch (tile_x) {
+        case 2: RTM_MERGED(2); break;
+        case 3: RTM_MERGED(3); break;
+        case 4: RTM_MERGED(4); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef RTM_MERGED
+    return hipGetLastError();
+}
+
+}  // namespace rtm_hip

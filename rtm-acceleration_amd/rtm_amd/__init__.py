@@ -201,10 +201,10 @@ class BatchEngine:
     each on its own grid, advanced together one time step per tick."""
 
     def __init__(self, device: int = 0, memory_budget: int = 0, max_active: int = 0,
-                 use_graph: bool = True, kernel_family: int = 0, tile_x: int = 0):
+                 use_graph: bool = True, kernel_family: int = 0, tile_x: int = 0, merge_groups: bool = False):
         self.lib = _capi.load()
         cfg = _capi.BatchConfig(int(memory_budget), int(max_active), int(bool(use_graph)), int(kernel_family),
-                                int(tile_x))
+                                int(tile_x), int(bool(merge_groups)))
         h = C.c_void_p()
         rc = self.lib.rtm_hip_batch_create(device, C.byref(cfg), C.byref(h))
         if rc != _capi.OK:

@@ -61,7 +61,7 @@ class KernelStats(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_size_t), ("max_active", C.c_int), ("use_graph", C.c_int),
-                ("kernel_family", C.c_int), ("tile_x", C.c_int)]
+                ("kernel_family", C.c_int), ("tile_x", C.c_int), ("merge_groups", C.c_int)]
 
 
 class Job(C.Structure):
@@ -80,7 +80,7 @@ class BatchStats(C.Structure):
                 ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int),
                 ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double),
                 ("graphs_captured", C.c_longlong), ("step_launches", C.c_longlong),
-                ("lean_launches", C.c_longlong)]
+                ("lean_launches", C.c_longlong), ("merged_launch", C.c_int)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

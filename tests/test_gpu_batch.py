@@ -68,12 +68,15 @@ def test_alone_matches_oracle_for_f32(jobs, alone):
         assert bits_equal(S_ref, alone[k][1])
 
 
+@pytest.mark.parametrize("merge", [False, True])
 @pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("max_active", [0, 3])
-def test_batch_equals_alone(jobs, alone, use_graph, max_active):
+def test_batch_equals_alone(jobs, alone, use_graph, max_active, merge):
+    """Per-group launches and the one merged launch per tick (every order
+    2-12 and snapshot width in one kernel) give the same bits."""
     import rtm_amd
-    print(f"case: batch of {len(jobs)} jobs graph={use_graph} max_active={max_active}", flush=True)
-    with rtm_amd.BatchEngine(0, max_active=max_active, use_graph=use_graph) as b:
+    print(f"case: batch of {len(jobs)} jobs graph={use_graph} max_active={max_active} merged={merge}", flush=True)
+    with rtm_amd.BatchEngine(0, max_active=max_active, use_graph=use_graph, merge_groups=merge) as b:
         ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
         b.run()
         st = b.stats()
@@ -87,6 +90,8 @@ def test_batch_equals_alone(jobs, alone, use_graph, max_active):
             assert st["ticks"] > longest
         if max_active:
             assert st["rebuilds"] > 2      # jobs joined mid-run
+        if merge:
+            assert st["merged_launch"] in (0, 1) and all(g["tile_x"] in (2, 3, 4) for g in st["groups"])
         for k, jid in enumerate(ids):
             state, lat = b.state(jid)
             assert state == rtm_amd.JOB_DONE and lat > 0
@@ -359,3 +364,19 @@ def test_lean_tick_patterns(hip_engine_cls, fmt, use_graph):
             I, S = b.result(jid)
             assert bits_equal(alone[k][0], I), f"job {k}: image max diff {np.abs(alone[k][0] - I).max()}"
             assert bits_equal(alone[k][1], S), f"job {k}: illumination differs"
+
+
+@pytest.mark.parametrize("tile", [2, 3, 4])
+def test_merged_launch_widths(jobs, alone, tile):
+    """The merged launch at each of its tile widths, pinned (DPP march)."""
+    import rtm_amd
+    print(f"case: merged launch, tile {tile}", flush=True)
+    with rtm_amd.BatchEngine(0, kernel_family=rtm_amd.STEP_MARCH_DPP, tile_x=tile, merge_groups=True) as b:
+        ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+        b.run(max_ticks=50, until_done=False)
+        assert b.stats()["merged_launch"] == 1          # jobs of several groups on the device
+        b.run()
+        for k, jid in enumerate(ids):
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}"
+            assert bits_equal(alone[k][1], S), f"job {k}"
