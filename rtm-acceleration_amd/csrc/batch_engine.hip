@@ -624,21 +624,19 @@ bool jobs_tick_lean(const rtm_hip_batch* b, int key, int ahead) {
 bool group_tick_lean(const rtm_hip_batch* b, int key, const Group& g, int ahead) {
     return g.family == rtm_hip::Family::kMarchDpp && jobs_tick_lean(b, key, ahead);
 }
-// The merged launch: lean when no job at all has snapshot work.
+// The merged launch picks the lean build per job on the device; for the
+// launch statistics a merged tick counts as lean when every job's is.
 bool merged_tick_lean(const rtm_hip_batch* b, int ahead) { return jobs_tick_lean(b, -1, ahead); }
 
 // The launch pattern of a run of `ticks` ticks starting `ahead` ticks after
 // the host's clocks: its length, then per tick and group '1' (lean) or '0'.
 std::string run_pattern(const rtm_hip_batch* b, int ticks, int ahead) {
-    std::string p = std::to_string(ticks) + (b->merged_on ? "m:" : ":");
-    for (int sub = 0; sub < ticks; ++sub) {
-        if (b->merged_on) {
-            p += merged_tick_lean(b, ahead + sub) ? '1' : '0';
-            continue;
-        }
+    // the merged launch is the same launch whatever its jobs' builds
+    if (b->merged_on) return std::to_string(ticks) + "m";
+    std::string p = std::to_string(ticks) + ":";
+    for (int sub = 0; sub < ticks; ++sub)
         for (const auto& [key, g] : b->groups)
             if (g.ntiles) p += group_tick_lean(b, key, g, ahead + sub) ? '1' : '0';
-    }
     return p;
 }
 
@@ -654,7 +652,7 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
         const Group& g = b->merged;
         for (int sub = 0; sub < ticks; ++sub) {
             hipError_t e = rtm_hip::launch_batch_merged(g.tile_x, b->d_jobs, b->d_targs, g.d_tile_job, g.ntiles, sub,
-                                                        b->stream, merged_tick_lean(b, ahead + sub));
+                                                        b->stream);
             if (e != hipSuccess) return b->hip_fail(e, "batched step launch");
         }
         hipError_t e = rtm_hip::launch_batch_advance(b->d_jobs, (int)b->active.size(), ticks, b->stream);
@@ -846,7 +844,7 @@ int autotune_merged(rtm_hip_batch* b) {
             for (int rep = 0; rep < (round == 0 ? reps + 2 : reps) && !rc; ++rep) {
                 if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
                 hipError_t le = rtm_hip::launch_batch_merged(g.tile_x, b->d_jobs, b->d_targs, g.d_tile_job, g.ntiles,
-                                                             0, b->stream, false);
+                                                             0, b->stream);
                 if (le != hipSuccess) rc = b->hip_fail(le, "autotune launch");
             }
             if (rc) break;
@@ -1046,8 +1044,14 @@ int run_solo(rtm_hip_batch* b, int n) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
-// Step launches of a run of ticks, and how many ran the lean build.
+// Step launches of a run of ticks, and how many ran the lean build (a
+// merged launch counts as lean when every job's build is).
 void count_launches(rtm_hip_batch* b, int ticks, int ahead) {
+    if (b->merged_on) {
+        b->stats.step_launches += ticks;
+        for (int sub = 0; sub < ticks; ++sub) b->stats.lean_launches += merged_tick_lean(b, ahead + sub) ? 1 : 0;
+        return;
+    }
     const std::string p = run_pattern(b, ticks, ahead);
     const size_t body = p.find(':') + 1;
     b->stats.step_launches += (long long)(p.size() - body);

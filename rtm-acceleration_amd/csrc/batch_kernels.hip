@@ -38,13 +38,15 @@ __global__ __launch_bounds__(64) void k_tick_args(const JobDesc* jobs, int njobs
              : (s16 && total == J.nt) ? kTickBookkeeping
                                       : kTickBackward;
     T.tile_first = J.tile_first;
-    T.build = 2 * J.half + (s16 ? 1 : 0);
+    bool lean = false;
     if (T.kind == kTickForward || T.kind == kTickBackward) {
         T.a = s16 ? job_args<true>(J, total, dry) : job_args<false>(J, total, dry);
+        lean = step_is_lean(T.kind == kTickForward ? Dir::Forward : Dir::Backward, T.a, s16);
     } else {                                 // bookkeeping reads the descriptor; idle nothing
         T.a = StepArgs{};
         T.a.dry = dry;
     }
+    T.build = 4 * J.half + (s16 ? 2 : 0) + (lean ? 1 : 0);
 }
 
 // Job clocks: every job has done `ticks` more steps.

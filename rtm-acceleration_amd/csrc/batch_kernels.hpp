@@ -73,9 +73,10 @@ struct TickArgs {
     StepArgs a;
     int kind;                    // TickKind
     int tile_first;              // the job's first tile in its group launch
-    int build;                   // 2 * half-width + (1 for 16-bit snapshots): the job's tile build in
-                                 // the merged launch (from the descriptor: a bookkeeping tick's
-                                 // step arguments are empty)
+    int build;                   // the job's tile build in the merged launch: 4 * half-width
+                                 // + 2 (16-bit snapshots) + 1 (lean: no snapshot work this tick);
+                                 // from the descriptor, as a bookkeeping tick's step arguments
+                                 // are empty
 };
 
 // Tick arguments of ticks 0..nsub-1 of the coming run for every job:
@@ -108,10 +109,11 @@ hipError_t launch_batch_step_s16(int half, Family family, int tile_x, const JobD
 // Tick `sub` of a run for every job of the batch in one launch, whatever
 // its stencil half-width or snapshot width: tile_job covers all jobs, each
 // with tile_x-column DPP march tiles, and each wave runs the march tile
-// built for its job's (half-width, snapshot width).  lean: no job has
-// snapshot work on this tick.  tile_x is one of merged_tile_widths().
+// built for its job's half-width and snapshot width, in its lean build when
+// the job has no snapshot work on this tick (TickArgs::build).  tile_x is
+// one of merged_tile_widths().
 hipError_t launch_batch_merged(int tile_x, const JobDesc* jobs, const TickArgs* targs, const int* tile_job,
-                               int ntiles, int sub, hipStream_t stream, bool lean);
+                               int ntiles, int sub, hipStream_t stream);
 inline std::vector<int> merged_tile_widths() { return {2, 3, 4}; }
 
 // FP64 stack update of a finished job: stack[(ix0+ix)*snz + iz0+iz] += image[ix*nz+iz]

@@ -3,11 +3,12 @@
 // The per-group launches (batch_launch.hpp) run jobs of one stencil
 // half-width and snapshot width each; with several groups a tick is several
 // launches, each with its own ramp and drain.  Here every job's tiles are in
-// one launch: a wave reads its job's half-width with its tick arguments and
-// runs, in a wave-uniform switch, the march tile built for that half-width
-// and snapshot width.  The kernel holds one tile body per (half-width,
-// snapshot width, direction); a wave only ever runs one, and the kernel's
-// registers are the largest body's.
+// one launch: a wave reads its job's build with its tick arguments and runs,
+// in a wave-uniform switch, the march tile built for that half-width and
+// snapshot width, lean when the job has no snapshot work on this tick.  The
+// kernel holds one tile body per (half-width, snapshot width, lean,
+// direction); a wave only ever runs one, and the kernel's registers are the
+// largest body's.
 #include "batch_launch.hpp"
 
 namespace rtm_hip {
@@ -25,7 +26,7 @@ __device__ __forceinline__ void merged_tile(const JobDesc& J, const StepArgs& a,
     else march_tile<H, TX, Dir::Backward, true, S16, LEAN>(a, a.coef, t, lane);
 }
 
-template <int TX, bool LEAN>
+template <int TX>
 __global__ __launch_bounds__(kBlockThreads) void k_batch_merged(const JobDesc* __restrict__ jobs,
                                                                 const TickArgs* __restrict__ targs,
                                                                 const int* __restrict__ tile_job, int ntiles, int sub) {
@@ -41,46 +42,37 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_merged(const JobDesc* _
     const int t = g - T.tile_first;
     const StepArgs& a = T.a;
     const JobDesc& J = jobs[jid];
-    // the job's (half-width, snapshot width), uniform over the wave; from
-    // the descriptor, since a bookkeeping tick's step arguments are empty
+    // the job's build, uniform over the wave (TickArgs::build)
+#define RTM_BUILDS(H)                                                                     \
+    case 4 * H: merged_tile<H, TX, false, false>(J, a, kind, t, lane); break;             \
+    case 4 * H + 1: merged_tile<H, TX, false, true>(J, a, kind, t, lane); break;          \
+    case 4 * H + 2: merged_tile<H, TX, true, false>(J, a, kind, t, lane); break;          \
+    case 4 * H + 3: merged_tile<H, TX, true, true>(J, a, kind, t, lane); break;
     switch (__builtin_amdgcn_readfirstlane(T.build)) {
-        case 2:  merged_tile<1, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 3:  merged_tile<1, TX, true, LEAN>(J, a, kind, t, lane); break;
-        case 4:  merged_tile<2, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 5:  merged_tile<2, TX, true, LEAN>(J, a, kind, t, lane); break;
-        case 6:  merged_tile<3, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 7:  merged_tile<3, TX, true, LEAN>(J, a, kind, t, lane); break;
-        case 8:  merged_tile<4, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 9:  merged_tile<4, TX, true, LEAN>(J, a, kind, t, lane); break;
-        case 10: merged_tile<5, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 11: merged_tile<5, TX, true, LEAN>(J, a, kind, t, lane); break;
-        case 12: merged_tile<6, TX, false, LEAN>(J, a, kind, t, lane); break;
-        case 13: merged_tile<6, TX, true, LEAN>(J, a, kind, t, lane); break;
+        RTM_BUILDS(1)
+        RTM_BUILDS(2)
+        RTM_BUILDS(3)
+        RTM_BUILDS(4)
+        RTM_BUILDS(5)
+        RTM_BUILDS(6)
         default: break;
     }
+#undef RTM_BUILDS
 }
 
 }  // namespace
 
 hipError_t launch_batch_merged(int tile_x, const JobDesc* jobs, const TickArgs* targs, const int* tile_job,
-                               int ntiles, int sub, hipStream_t stream, bool lean) {
+                               int ntiles, int sub, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     const dim3 block(kBlockThreads);
     const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves));
-#define RTM_MERGED(TX)                                                                                         \
-    do {                                                                                                       \
-        if (lean) hipLaunchKernelGGL((k_batch_merged<TX, true>), grid, block, 0, stream, jobs, targs, tile_job, \
-                                     ntiles, sub);                                                             \
-        else hipLaunchKernelGGL((k_batch_merged<TX, false>), grid, block, 0, stream, jobs, targs, tile_job,     \
-                                ntiles, sub);                                                                  \
-    } while (0)
     switch (tile_x) {
-        case 2: RTM_MERGED(2); break;
-        case 3: RTM_MERGED(3); break;
-        case 4: RTM_MERGED(4); break;
+        case 2: hipLaunchKernelGGL((k_batch_merged<2>), grid, block, 0, stream, jobs, targs, tile_job, ntiles, sub); break;
+        case 3: hipLaunchKernelGGL((k_batch_merged<3>), grid, block, 0, stream, jobs, targs, tile_job, ntiles, sub); break;
+        case 4: hipLaunchKernelGGL((k_batch_merged<4>), grid, block, 0, stream, jobs, targs, tile_job, ntiles, sub); break;
         default: return hipErrorInvalidValue;
     }
-#undef RTM_MERGED
     return hipGetLastError();
 }
 

@@ -172,7 +172,7 @@ __host__ __device__ inline int pair_first(int nt, int store, bool pending_top) {
 // nor the maxima or codes of a 16-bit one (the reduction of the maxima, red_in,
 // runs in the lean build too).  The same tests as the kernels' own
 // (snap_ctx), on the host.
-inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
+__host__ __device__ inline bool step_is_lean(Dir d, const StepArgs& a, bool s16) {
     if (d == Dir::Forward) return a.snap_out == nullptr && a.wave_max == nullptr && a.quant_out == nullptr;
     return s16 ? (a.snap16_in == nullptr && a.pending_field == nullptr) : a.snap_in == nullptr;
 }
