@@ -76,7 +76,7 @@ struct rtm_hip_engine {
     float* d_wx = nullptr;
     float* d_wz = nullptr;
     float* d_wavelet = nullptr;
-    float* d_fld[3] = {nullptr, nullptr, nullptr};
+    float* d_fld[2] = {nullptr, nullptr};     // p^{n-1} (overwritten by p^{n+1}) and p^n
     float* d_hold[2] = {nullptr, nullptr};   // fields held for paired imaging (pair_step)
     float* d_snap = nullptr;         // FP32 snapshots
     uint16_t* d_snap16 = nullptr;    // FP16/BF16 snapshot codes
@@ -466,28 +466,34 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         phase_ev = take_event(h);
         (void)hipEventRecord(phase_ev.start, h->stream);
     }
-    int ia = 0, ib = 1, ic = 2;   // p_prev, p_cur, p_next
-    // The field written by backward step s: the rotation's buffer (s + 2) % 3,
-    // or on a Hold step of paired imaging a hold buffer, which the following
-    // steps read as p^n, p^{n-1} and the Pair step as the held field.  (The
-    // buffers start zeroed, so steps 0 and 1 read zeros as before.)
+    // Two wavefield buffers: each step writes p^{n+1} over p^{n-1}, in place.
+    // A step reads p^{n-1} only at its own output cells, in the lane that
+    // writes p^{n+1} there, and before that write (the value feeds the
+    // store), so no other wave needs the old value; the step's working set is
+    // three fields instead of four, which on a grid of a few million cells
+    // stays within the L2 caches from one launch to the next.  On a Hold step
+    // of paired imaging the field goes to a hold buffer instead, which the
+    // following steps read as p^n and p^{n-1} and the Pair step as the held
+    // field; the step after it, whose p^{n-1} is held, writes the rotation
+    // buffer that does not hold its p^n.  Both buffers start zeroed, so steps
+    // 0 and 1 read zeros.
+    float* const r0 = h->d_fld[0];
+    float* const r1 = h->d_fld[1];
+    float* prev = r0;
+    float* cur = r1;
     const int first = rtm_hip::pair_first(h->nt, h->store, false);
-    auto bwd_out = [&](int s) -> float* {
-        const rtm_hip::PairStep p = rtm_hip::pair_step(h->nt - 1 - s, h->store, first);
-        return (snapshots && p.role == rtm_hip::PairRole::Hold) ? h->d_hold[p.hold] : h->d_fld[(s + 2) % 3];
-    };
     for (int step = 0; step < h->nt; ++step) {
         const int it = dir == Dir::Forward ? step : h->nt - 1 - step;
         a.it = it;
-        if (dir == Dir::Forward) {
-            a.p_prev = h->d_fld[ia];
-            a.p_cur = h->d_fld[ib];
-            a.p_next = h->d_fld[ic];
-        } else {
-            a.p_prev = step >= 2 ? bwd_out(step - 2) : h->d_fld[step % 3];
-            a.p_cur = step >= 1 ? bwd_out(step - 1) : h->d_fld[1];
-            a.p_next = bwd_out(step);
+        float* out = prev;
+        if (dir == Dir::Backward && snapshots) {
+            const rtm_hip::PairStep p = rtm_hip::pair_step(it, h->store, first);
+            if (p.role == rtm_hip::PairRole::Hold) out = h->d_hold[p.hold];
+            else if (prev != r0 && prev != r1) out = cur == r0 ? r1 : r0;
         }
+        a.p_prev = prev;
+        a.p_cur = cur;
+        a.p_next = out;
         if (snapshots) set_snapshot_args(h, dir, it, a);
 
         const bool sample = h->timing_every > 0 && (h->launches[k] % h->timing_every) == 0;
@@ -497,11 +503,8 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
         if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
         if (sample) h->ev_pending.push_back({k, ev, 1});
         h->launches[k] += 1;
-        // rotate: p_prev <- p_cur, p_cur <- p_next, p_next <- old p_prev
-        const int old = ia;
-        ia = ib;
-        ib = ic;
-        ic = old;
+        prev = cur;
+        cur = out;
     }
     if (phase_ev.start) {
         (void)hipEventRecord(phase_ev.stop, h->stream);
@@ -509,8 +512,8 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
     }
     // 16-bit stored steps among the last two have no step it+2 in the loop:
     // their max from the slices (step nt-2) or the per-wave maxima (step
-    // nt-1), then their codes from the field (p_cur / p_prev after the loop's
-    // last rotation hold the fields of steps nt-1 / nt-2)
+    // nt-1), then their codes from the field (after the loop, cur / prev hold
+    // the fields of steps nt-1 / nt-2)
     if (dir == Dir::Forward && snapshots && h->snap_fmt != RTM_HIP_SNAP_F32) {
         const int nparts = g.family == rtm_hip::Family::kLdsTile ? g.ntiles * rtm_hip::kWaves : g.ntiles;
         for (int it = std::max(0, h->nt - 2); it < h->nt; ++it) {
@@ -524,7 +527,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
                                                              std::min(rtm_hip::kReduceWaves, nparts),
                                                              h->d_snap_max + slot, h->stream);
             if (e == hipSuccess)
-                e = rtm_hip::launch_quantise(h->d_fld[last ? ib : ia], h->ldz, h->nb, h->nx, h->nz,
+                e = rtm_hip::launch_quantise(last ? cur : prev, h->ldz, h->nb, h->nx, h->nz,
                                              static_cast<rtm_hip::SnapFmt>(h->snap_fmt), h->d_snap_max + slot,
                                              h->d_snap16 + slot * h->n_int, h->stream);
             if (e != hipSuccess) return h->hip_fail(e, "snapshot quantisation");
@@ -595,9 +598,10 @@ int autotune(rtm_hip_engine* h, Dir dir) {
             const int count = (round == 0 ? kWarm : 0) + kTimed;
             for (int i = 0; i < count && !rc; ++i, ++n) {
                 if (i == count - kTimed) (void)hipEventRecord(ev.start, h->stream);
-                a.p_prev = h->d_fld[n % 3];
-                a.p_cur = h->d_fld[(n + 1) % 3];
-                a.p_next = h->d_fld[(n + 2) % 3];
+                // the loop's in-place update over two buffers
+                a.p_prev = h->d_fld[n & 1];
+                a.p_cur = h->d_fld[(n + 1) & 1];
+                a.p_next = h->d_fld[n & 1];
                 // slot 0 (and its neighbour for 16-bit quantisation) stand in
                 // for the real slots; the propagation rewrites them
                 set_snapshot_args(h, dir, n % std::min(h->nt, 2 * h->store), a);
@@ -856,7 +860,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
 
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-        const size_t need = h->snap_bytes + 7 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float) +
+        const size_t need = h->snap_bytes + 6 * h->n_fld * sizeof(float) + 2 * h->n_int * sizeof(float) +
                             (h->image_stack ? 2 * h->n_int * sizeof(double) : 0);
         if (need > free_b)
             return h->fail(RTM_HIP_ERR_MEMORY,
@@ -870,7 +874,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (!rc) rc = h->alloc(&h->d_wx, (size_t)h->nxe + rtm_hip::kGuardCols, "wx");
     if (!rc) rc = h->alloc(&h->d_wz, (size_t)h->ldz, "wz");
     if (!rc) rc = h->alloc(&h->d_wavelet, (size_t)nt, "wavelet");
-    for (int b = 0; b < 3 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");
+    for (int b = 0; b < 2 && !rc; ++b) rc = h->alloc(&h->d_fld[b], h->n_fld, "wavefield");
     for (int b = 0; b < 2 && !rc; ++b) rc = h->alloc(&h->d_hold[b], h->n_fld, "held wavefield");
     if (h->snap_fmt == RTM_HIP_SNAP_F32) {
         if (!rc) rc = h->alloc(&h->d_snap, h->nsnap * h->n_int, "snapshot store");
@@ -916,7 +920,7 @@ int rtm_hip_setup(rtm_hip_engine* h, const rtm_hip_grid* g, const float* v, cons
     if (e == hipSuccess && h->d_snap_max) e = hipMemset(h->d_snap_max, 0, (size_t)h->nsnap * sizeof(float));
     if (e == hipSuccess && h->d_stack_i) e = hipMemset(h->d_stack_i, 0, h->n_int * sizeof(double));
     if (e == hipSuccess && h->d_stack_s) e = hipMemset(h->d_stack_s, 0, h->n_int * sizeof(double));
-    for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMemset(h->d_fld[b], 0, h->n_fld * sizeof(float));
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemset(h->d_fld[b], 0, h->n_fld * sizeof(float));
     // a step writes output cells only: the outer ring and the guard columns
     // of a hold buffer are read as p^n / p^{n-1} and must stay zero
     for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemset(h->d_hold[b], 0, h->n_fld * sizeof(float));

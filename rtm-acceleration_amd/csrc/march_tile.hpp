@@ -747,16 +747,23 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     int e0 = 0, e1 = 0;
     if constexpr (!S) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
 
-    // every load of the tile, issued before any use
+    // Every load of the tile, issued before any use, in the order the
+    // columns use them: the depth taper, p^n columns 0 .. 2H-1, then per
+    // output column r the p^n column r + 2H that completes its window and
+    // its own p^{n-1} and (v dt)^2.  The vector memory counter retires in
+    // order, so the arithmetic of column r waits only for the loads up to
+    // its own, and overlaps the latency of the later columns' loads.
+    const float wzv = gbl(a.wz)[s.z0 + s.lz];
     const global_t<const float>* pc = field_col(a.p_cur, a, s, x0 - H);
-    float q[NQ];
-#pragma unroll
-    for (int j = 0; j < NQ; ++j) q[j] = pc[(ptrdiff_t)j * a.ldz + s.lz];
-    ColumnOps ops[TX];
     const global_t<const float>* pp = field_col(a.p_prev, a, s, x0);
     const global_t<const float>* pv2 = field_col(a.vdt2, a, s, x0);
+    float q[NQ];
+    ColumnOps ops[TX];
+#pragma unroll
+    for (int j = 0; j < 2 * H; ++j) q[j] = pc[(ptrdiff_t)j * a.ldz + s.lz];
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
+        q[r + 2 * H] = pc[(ptrdiff_t)(r + 2 * H) * a.ldz + s.lz];
         ops[r].pv = pp[(ptrdiff_t)r * a.ldz + s.lz];
         ops[r].vv = pv2[(ptrdiff_t)r * a.ldz + s.lz];
         ops[r].wx = sld(a.wx + x0 + r);
@@ -765,7 +772,6 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     }
     if constexpr (D == Dir::Backward && !LEAN)
         if (sc.imaging) strip_imaging_ops<TX, S>(a, s, ops);
-    const float wzv = gbl(a.wz)[s.z0 + s.lz];
     if constexpr (S) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
     const TileEntries te = load_entries<D>(a, e0, e1, lane);
     snap_scales<S>(a, sc, lane);

@@ -18,8 +18,11 @@
 //   p^{n+1} = ((2 p^n - p^{n-1}*w) + vdt2*lap) * w,   w = wx[ix]*wz[iz]
 // The reference damps p^n in place after the stencil (forward.cpp:56-63); that
 // multiply is applied here when the buffer is read back as p^{n-1} on the next
-// step (p_prev*w), so no thread ever writes a cell another workgroup reads in
-// the same launch.  Injection and recording cells are interior cells, where
+// step (p_prev*w), so no step writes a cell of p^n, which other workgroups
+// read as their halo.  p^{n-1} is used only at a step's own output cells, in
+// the lane that writes p^{n+1} there, so p_next may be p_prev (an in-place
+// update): march strips also load p^{n-1} in their halo rows, but never use
+// those values.  Injection and recording cells are interior cells, where
 // w == 1.0f exactly, so injecting after the damped write is the reference's
 // inject-then-damp.  Duplicate receiver cells add in ascending receiver order
 // through a per-tile entry table, which is the reference's sequential order.
