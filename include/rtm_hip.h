@@ -355,7 +355,10 @@ int         rtm_hip_batch_submit(rtm_hip_batch* b, const rtm_hip_job* job, long 
 /* Admits queued jobs and runs ticks.  until_done = 1: until no job is queued
  * or active (or max_ticks > 0 ticks have run).  until_done = 0: at most
  * max_ticks ticks (max_ticks <= 0: until the next job finishes).  Jobs that
- * finish are retired (image downloaded, stack updated, memory freed). */
+ * finish are retired (image downloaded, stack updated, memory freed).  A call
+ * with until_done = 0 that leaves jobs running returns with its last ticks
+ * still queued on the device, once the ticks the previous such call left
+ * are done, so a serving loop keeps the device busy between calls. */
 int         rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done);
 
 /* State (RTM_HIP_JOB_*) and submit-to-done latency (-1 until done). */

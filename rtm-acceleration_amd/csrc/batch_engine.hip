@@ -143,6 +143,11 @@ struct rtm_hip_batch {
     hipStream_t stream = nullptr;
     hipStream_t group_streams[kGroupStreams - 1] = {};   // groups 1, 2, ... of a run (group 0: stream)
     hipEvent_t fork_ev = nullptr;                       // the run's tick arguments are ready
+    // the ticks a rtm_hip_batch_run call with jobs left running leaves on the
+    // device (the next such call waits for them)
+    hipEvent_t chunk_ev[2] = {};
+    int chunk_cur = 0;
+    bool chunk_pending = false;
     hipEvent_t join_ev[kGroupStreams - 1] = {};         // a group stream's launches of the run are done
     std::string err;
     rtm_hip_batch_config cfg{};
@@ -1168,6 +1173,7 @@ int rtm_hip_batch_create(int device, const rtm_hip_batch_config* cfg, rtm_hip_ba
         return RTM_HIP_ERR_DEVICE;
     }
     bool ok = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (auto& ev : b->chunk_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < kGroupStreams - 1 && ok; ++k)
         ok = hipStreamCreateWithFlags(&b->group_streams[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&b->join_ev[k], hipEventDisableTiming) == hipSuccess;
@@ -1204,6 +1210,8 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
         if (b->join_ev[k]) (void)hipEventDestroy(b->join_ev[k]);
     }
     if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    for (hipEvent_t ev : b->chunk_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1331,7 +1339,20 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         if (!until_done && (max_ticks <= 0 || ran >= max_ticks)) break;
         if (until_done && max_ticks > 0 && ran >= max_ticks) break;
     }
-    hipError_t e = hipStreamSynchronize(b->stream);
+    // A call that leaves jobs running (until_done = 0) returns with its last
+    // ticks still queued on the device and waits only for the ticks the
+    // previous such call left: the device has work while the caller admits
+    // new jobs between calls.  Otherwise, and on an error, it waits for all.
+    hipError_t e;
+    if (rc || until_done || b->active.empty()) {
+        e = hipStreamSynchronize(b->stream);
+        b->chunk_pending = false;
+    } else {
+        e = hipEventRecord(b->chunk_ev[b->chunk_cur], b->stream);
+        if (e == hipSuccess && b->chunk_pending) e = hipEventSynchronize(b->chunk_ev[b->chunk_cur ^ 1]);
+        b->chunk_pending = true;
+        b->chunk_cur ^= 1;
+    }
     b->stats.run_seconds += now_s() - t0;
     if (rc) return rc;
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "batch run");

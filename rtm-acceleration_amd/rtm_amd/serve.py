@@ -140,6 +140,7 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
         waiting: list[int] = []                          # arrived, not yet admitted, in arrival order
         nxt = 0
         chunk = 16
+        t_ret, prev_ran = None, 0          # return time and ticks of the previous run() call
         t0 = time.perf_counter()
         while nxt < len(pending) or out or waiting:
             now = time.perf_counter() - t0
@@ -155,14 +156,19 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
                 jid = b.submit_shot(mids[r.key], r.shot)
                 out[jid] = (waiting.pop(0), (time.perf_counter() - t0) - r.arrival, cells)
             if not out:
+                t_ret = None
                 time.sleep(max(0.0, requests[pending[nxt]].arrival - (time.perf_counter() - t0)))
                 continue
-            ticks0, c0 = b.stats()["ticks"], time.perf_counter()
+            ticks0 = b.stats()["ticks"]
             b.run(max_ticks=chunk, until_done=False)
-            dt_run = time.perf_counter() - c0
+            c1 = time.perf_counter()
             ran = b.stats()["ticks"] - ticks0
-            if ran > 0 and dt_run > 0:      # keep a chunk near chunk_s of device time
-                chunk = int(np.clip(chunk * chunk_s / (dt_run / ran * chunk), 4, 4096))
+            # run() returns once the previous call's ticks are done and leaves
+            # its own queued, so the time between two returns is the device
+            # time of the previous call's ticks: keep a chunk near chunk_s
+            if t_ret is not None and prev_ran > 0 and c1 > t_ret:
+                chunk = int(np.clip(chunk_s * prev_ran / (c1 - t_ret), 4, 4096))
+            t_ret, prev_ran = c1, ran
             for jid in list(out):
                 state, lat = b.state(jid)
                 if state == 2:
