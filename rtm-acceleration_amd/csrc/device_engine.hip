@@ -549,7 +549,7 @@ int run_loop(rtm_hip_engine* h, Dir dir, const rtm_hip_shot* s, bool record,
 }
 
 // Autotuner: times every step-kernel family and tile width on this grid for
-// one direction (best of 3 x 16 launches each, no injection entries, imaging or snapshot
+// one direction (best of 3 x 48 launches each, no injection entries, imaging or snapshot
 // traffic on every store-th launch as in the real loop) and keeps the
 // fastest.  Runs once per setup and direction, before the direction's first
 // propagation; the wavefields are reset afterwards by the propagation itself.
@@ -587,9 +587,9 @@ int autotune(rtm_hip_engine* h, Dir dir) {
     float best_ms = 1e30f;
     rtm_hip::StepGeometry best = h->geometry(k);
     int rc = RTM_HIP_OK;
-    // per candidate: 4 warm launches, then the best of 3 rounds of 16 timed
+    // per candidate: 4 warm launches, then the best of 3 rounds of 48 timed
     // launches (the minimum is the least noisy estimate of a kernel's time)
-    constexpr int kWarm = 4, kRounds = 3, kTimed = 16;
+    constexpr int kWarm = 4, kRounds = 3, kTimed = 48;
     auto time_geometry = [&](const rtm_hip::StepGeometry& g) -> float {
         a.ntz = g.ntz; a.ntz_magic = g.ntz_magic; a.ntx = g.ntx; a.ntiles = g.ntiles;
         float cand_ms = 1e30f;
@@ -621,13 +621,14 @@ int autotune(rtm_hip_engine* h, Dir dir) {
         return cand_ms;
     };
     // stage 1: mapping and tile width at full occupancy; candidates are
-    // tried in order of preference and a later one must win by 5 % so that
-    // timing noise does not flip the choice
+    // tried in order of preference and a later one must win by 2 % so that
+    // timing noise does not flip the choice (48 launches per round keep the
+    // rounds' spread below that)
     for (const Candidate& c : cands) {
         const rtm_hip::StepGeometry g = rtm_hip::step_geometry(c.family, h->half, c.tile_x, h->nxe, h->nze, h->ldz);
         const float ms = time_geometry(g);
         if (rc) break;
-        if (ms < 0.95f * best_ms) {
+        if (ms < 0.98f * best_ms) {
             best_ms = ms;
             best = g;
         }
