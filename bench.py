@@ -442,6 +442,8 @@ def main():
     ap.add_argument("--batch-family", type=int, default=0,
                     help="batch step kernel (0 = autotuned, 1 march, 2 DPP march, 3 stream)")
     ap.add_argument("--batch-tile", type=int, default=0, help="batch step kernel tile width (0 = default)")
+    ap.add_argument("--snapshots", choices=("f32", "f16", "bf16"), default="f32",
+                    help="snapshot format of the plain engine's line (the workload's is f32)")
     ap.add_argument("--batch-snapshots", choices=("f32", "f16", "bf16"), default="f32",
                     help="snapshot format of the batched comparison's jobs")
     ap.add_argument("--batch-merge", action="store_true",
@@ -492,10 +494,12 @@ def main():
     NB = 0 if args.no_batched else max(K, args.batched_shots)
     model, params, nt, dt, shots = workload(max(K, NB) + W, args.nt)
     copy_peak = rtm_amd.bandwidth_probe(0, args.probe_bytes, 5)
-    n_ext, bytes_shot, fwd_b, bwd_b = algorithmic_bytes(model, params, nt)
+    fmt = ("f32", "f16", "bf16").index(args.snapshots)
+    snap_b = 4 if fmt == 0 else 2
+    n_ext, bytes_shot, fwd_b, bwd_b = algorithmic_bytes(model, params, nt, snap_b)
 
     eng = rtm_amd.HipEngine(0)
-    eng.setup(model, params, nt, dt)
+    eng.setup(model, params, nt, dt, snapshot_format=fmt)
     eng.set_step_kernel(args.family, args.tile)
     print(f"bench: device {eng.info()['device_name']}, modelling {len(shots)} shots", file=sys.stderr, flush=True)
     for s in shots:
@@ -533,7 +537,7 @@ def main():
     fwd_ms = ks["sampled_ms"][0] / max(ks["sampled"][0], 1)
     bwd_ms = ks["sampled_ms"][1] / max(ks["sampled"][1], 1)
     achieved = bwd_b / (bwd_ms * 1e-3) * 1e-9
-    bwd_exec = executed_backward_bytes(model, params, nt)
+    bwd_exec = executed_backward_bytes(model, params, nt, snap_b)
     shot_bytes_rate = bytes_shot * K / elapsed * 1e-9
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1,
@@ -541,7 +545,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded faulted-layer model; traces modelled on the GPU)",
         "config": {"workload": "2301x751 @4 m, nt 8000, dt 0.35 ms, order 8, nb 60, store 4, "
-                               "767 receivers, plain per-shot engine, FP32 snapshots",
+                               f"767 receivers, plain per-shot engine, {args.snapshots.upper()} snapshots",
                    "engine": "hip",
                    "step_kernel": {d: {"family": ["lds-tile", "march", "march-dpp", "stream"][info["step_family"][k]],
                                        "tile_x": info["tile_x"][k], "autotuned": bool(info["autotuned"][k]),
@@ -553,7 +557,8 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
         "whole_shot_hbm_frac": round(shot_bytes_rate / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_per_launch() if fmt == 0 else None,   # measured with FP32 snapshots
                      "copy_probe_gbs": round(copy_peak, 1),
                      "frac_of_copy_probe": round(achieved / copy_peak, 4) if copy_peak else None,
                      "kernel": "backward fused step (stencil + sponge + injection + imaging), order 8; "
