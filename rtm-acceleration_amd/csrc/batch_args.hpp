@@ -9,15 +9,24 @@
 
 namespace rtm_hip {
 
+// Two field buffers: step s of a phase writes p^{n+1} over its own p^{n-1},
+// in place (a step reads p^{n-1} only at its own output cells, in the lane
+// that writes p^{n+1} there, before that write), so the buffer of step s is
+// fld[(base + s) & 1] and steps s and s - 2 share it.
 __host__ __device__ inline float* rotate(const JobDesc& J, int r) {
-    return r == 0 ? J.fld[0] : (r == 1 ? J.fld[1] : J.fld[2]);
+    return (r & 1) ? J.fld[1] : J.fld[0];
 }
 
 // The field written by backward step s of job J: the rotation's buffer, or
-// on a Hold step of paired imaging a hold buffer (pair_step).
+// on a Hold step of paired imaging a hold buffer (pair_step).  The step two
+// after a Hold step, whose p^{n-1} is held, still writes buffer
+// (bwd_base + s) & 1: the one that does not hold its p^n, last written by
+// step s - 4.  bwd_base = nt & 1 keeps the forward phase's last field
+// (buffer (nt - 1) & 1) intact through backward step 0, which images it
+// when it is a pending 16-bit snapshot.
 __host__ __device__ inline float* bwd_out(const JobDesc& J, int s) {
     const PairStep p = pair_step(J.nt - 1 - s, J.store, J.pair_first);
-    return p.role == PairRole::Hold ? J.hold[p.hold] : rotate(J, (J.bwd_base + s + 2) % 3);
+    return p.role == PairRole::Hold ? J.hold[p.hold] : rotate(J, J.bwd_base + s);
 }
 
 // Step arguments of job J after `total` of its steps (forward steps
@@ -30,21 +39,23 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     constexpr bool wide = !S16;
     const int fwd = total < J.nt;
     const int s = fwd ? total : total - J.nt - (wide ? 0 : 1);
-    const int base = fwd ? 0 : J.bwd_base;
     // a phase's first steps read p^{n-1} (steps 0, 1) and p^n (step 0) as
     // zero: from the model's zero field, so the march tile loads them as any
     // other field (the flags serve the other mappings)
     a.zero_prev = s <= 1;
     a.zero_cur = s == 0;
     if (fwd) {
-        a.p_prev = a.zero_prev ? J.zero : rotate(J, (base + s) % 3);
-        a.p_cur = a.zero_cur ? J.zero : rotate(J, (base + s + 1) % 3);
-        a.p_next = rotate(J, (base + s + 2) % 3);
+        a.p_prev = a.zero_prev ? J.zero : rotate(J, s);
+        a.p_cur = a.zero_cur ? J.zero : rotate(J, s + 1);
+        a.p_next = rotate(J, s);
     } else {
         a.p_prev = a.zero_prev ? J.zero : bwd_out(J, s - 2);
         a.p_cur = a.zero_cur ? J.zero : bwd_out(J, s - 1);
         a.p_next = bwd_out(J, s);
     }
+    // a dry step (the autotuner's timing launches on live jobs) must not
+    // overwrite the job's p^{n-1} in place, nor a held field
+    if (dry) a.p_next = J.fld[2];
     a.coef = J.coef;
     a.vdt2 = J.vdt2;
     a.wx = J.wx;
@@ -99,7 +110,7 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
     a.quant_max_out = quant ? J.snap_max + (it - 2) / st : nullptr;
     a.parts_cap = J.parts_cap;
     a.snap16_max = (!wide && imaging && !pending) ? J.snap_max + slot : nullptr;
-    a.pending_field = pending ? rotate(J, J.bwd_base) : nullptr;
+    a.pending_field = pending ? rotate(J, J.nt - 1) : nullptr;
     a.snap16_in = (!wide && imaging && !pending) ? J.snap16 + slot * J.n_int : nullptr;
     return a;
 }

@@ -323,7 +323,7 @@ size_t job_footprint(const Model& m, size_t nrec, size_t ntraces) {
     const size_t snap_elem = m.params.snapshot_format == RTM_HIP_SNAP_F32 ? 4 : 2;
     const size_t tiles_bound = (size_t)(m.nxe / 2 + 2) * (m.nze / 52 + 2) + 1;   // tile_x >= 2, strip >= 52
     size_t bytes = 0;
-    bytes += 5 * m.n_fld * sizeof(float);                   // three wavefields, two held for paired imaging
+    bytes += 5 * m.n_fld * sizeof(float);                   // two wavefields, a dry-step output, two held
     bytes += (size_t)m.nsnap * m.n_int * snap_elem;         // snapshot store
     if (snap_elem == 2)                                     // spare word, scales and their bookkeeping
         bytes += ((size_t)m.nsnap + 2 * tiles_bound + 2 * rtm_hip::kReduceWaves + 5) * sizeof(float);
@@ -460,7 +460,7 @@ int admit_job(rtm_hip_batch* b, Job& j) {
     d.half = m.half; d.nt = m.nt; d.store = m.params.store_interval; d.snap_fmt = m.params.snapshot_format;
     d.coef = m.coef;
     d.n_int = (long long)m.n_int;
-    d.bwd_base = (m.nt + 1) % 3;            // buffer of the forward phase's last field
+    d.bwd_base = m.nt & 1;                  // not the buffer of the forward phase's last field
     d.vdt2 = m.vdt2; d.wx = m.wx; d.wz = m.wz; d.wavelet = m.wavelet;
     d.zero = m.zero;
     // 16-bit jobs image their top stored step from its pending field at
@@ -698,8 +698,8 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
 }
 
 // Kernel family, tile width and wavefronts per CU per group, picked by
-// timing dry steps (no clock advance, no image or snapshot update: the next
-// real step rewrites the same p^{n+1}) on the live jobs; the launches are
+// timing dry steps (no clock advance, no image or snapshot update, p^{n+1}
+// to a buffer of its own, so the live state is untouched) on the live jobs; the launches are
 // long, so a candidate replaces the best when it is 2 % faster.
 int autotune_merged(rtm_hip_batch* b);
 

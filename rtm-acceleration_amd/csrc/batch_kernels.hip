@@ -6,10 +6,11 @@
 // Phase boundaries need no reset pass: in the first step of a phase p^n and
 // p^{n-1} are read as zero, and in the second p^{n-1} is (their buffers still
 // hold the other phase's data); the reference zeroes all three buffers
-// instead (src/rtm/backward.cpp:46-48), which gives the same arithmetic.  The
-// backward phase rotates from the buffer that holds the forward phase's last
-// field, so a 16-bit snapshot of that last step can be taken at backward step
-// 0 straight from it.
+// instead (src/rtm/backward.cpp:46-48), which gives the same arithmetic.  A
+// job's two field buffers alternate, each step writing over its own p^{n-1}
+// (batch_args.hpp); the backward phase starts in the buffer that does not
+// hold the forward phase's last field, so a 16-bit snapshot of that last
+// step can be taken at backward step 0 straight from it.
 //
 // The step kernel itself is in batch_launch.hpp, built once for jobs with
 // FP32 snapshots and once for jobs with 16-bit ones; this file holds the

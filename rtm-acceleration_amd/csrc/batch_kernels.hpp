@@ -25,7 +25,7 @@ struct JobDesc {
     int ntz, ntx, ntiles;        // this job's tile grid (march mapping)
     unsigned ntz_magic;          // tile_div_magic(ntz)
     int tile_first;              // its first tile in the group launch
-    int bwd_base;                // buffer rotation base of the backward phase
+    int bwd_base;                // rotation parity of the backward phase's first output
     // clock, advanced on the device after every run of ticks
     int done;                    // steps done of `steps`
     int steps;                   // forward nt, then (16-bit snapshots) one
@@ -38,7 +38,8 @@ struct JobDesc {
     const float* wavelet;
     float* amps;                 // receiver amplitudes in backward-entry order (StepArgs::amps)
     int amp_count;
-    float* fld[3];
+    float* fld[3];               // p^{n-1} (overwritten in place by p^{n+1}), p^n, and the
+                                 // output of dry (timing) steps, which leave the state intact
     float* hold[2];              // fields held for paired imaging (pair_step)
     int pair_first;              // pair_first() of the backward phase
     const float* zero;           // a field of zeros (the model's), read as p^{n-1} / p^n on a phase's first steps

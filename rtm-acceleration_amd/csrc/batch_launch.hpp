@@ -29,7 +29,7 @@ __device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int la
         const float m = parts_max(J.slice_max + p * kReduceWaves, J.part_counts + 2 + p, lane);
         const int k = snapshot_scale_exponent(m);
         if (t == 0 && lane == 0) gbl(J.snap_max)[slot] = m;
-        const float* f = rotate(J, nt % 3);          // buffer of forward step nt-2's field
+        const float* f = rotate(J, nt - 2);          // buffer of forward step nt-2's field
         uint16_t* out = J.snap16 + (ptrdiff_t)slot * J.n_int;
         const int tz = t % J.ntz, tx = t / J.ntz;
         const int z = tz * TZ + lane;

@@ -34,8 +34,8 @@ int main(int argc, char** argv) {
         J.snap_fmt = fmt;
         J.n_int = kInt;
         J.parts_cap = kCap;
-        J.bwd_base = (nt + 1) % 3;
-        // fields 0-2, hold buffers 3-4, the zero field 5 (fld() below)
+        J.bwd_base = nt & 1;
+        // fields 0-1, the dry-step output 2, hold buffers 3-4, the zero field 5 (fld() below)
         for (int k = 0; k < 3; ++k) J.fld[k] = (float*)(uintptr_t)(0x1000000u * (k + 1));
         for (int k = 0; k < 2; ++k) J.hold[k] = (float*)(uintptr_t)(0x1000000u * (k + 4));
         J.zero = (const float*)(uintptr_t)(0x1000000u * 6);
@@ -54,6 +54,7 @@ int main(int argc, char** argv) {
                 continue;
             }
             const StepArgs a = s16 ? rtm_hip::job_args<true>(J, total, 0) : rtm_hip::job_args<false>(J, total, 0);
+            const StepArgs ad = s16 ? rtm_hip::job_args<true>(J, total, 1) : rtm_hip::job_args<false>(J, total, 1);
             auto fld = [&](const float* p) { return p ? (long long)((uintptr_t)p / 0x1000000u) - 1 : -1LL; };
             std::printf(
                 "{\"case\":\"%s\",\"total\":%d,\"kind\":\"%s\",\"it\":%d,\"prev\":%lld,\"cur\":%lld,\"next\":%lld,"
@@ -61,7 +62,7 @@ int main(int argc, char** argv) {
                 "\"red_in\":%lld,\"red_count\":%lld,\"red_out\":%lld,\"red_out_count\":%lld,"
                 "\"qparts\":%lld,\"qcount\":%lld,\"qout\":%lld,\"qmax\":%lld,"
                 "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld,\"lean\":%d,"
-                "\"hold\":%lld,\"snap_in2\":%lld,\"s16in2\":%lld,\"s16max2\":%lld}\n",
+                "\"hold\":%lld,\"snap_in2\":%lld,\"s16in2\":%lld,\"s16max2\":%lld,\"dry_next\":%lld}\n",
                 argv[c], total, total < nt ? "fwd" : "bwd", a.it, fld(a.p_prev), fld(a.p_cur), fld(a.p_next),
                 a.zero_prev, a.zero_cur, idx(a.snap_out, 0x10000000u, kInt * 4), idx(a.snap_in, 0x10000000u, kInt * 4),
                 idx(a.wave_max, 0x30000000u, kCap * 4), idx(a.wave_count, 0x50000000u, 4),
@@ -72,7 +73,7 @@ int main(int argc, char** argv) {
                 idx(a.snap16_in, 0x20000000u, kInt * 2), idx(a.snap16_max, 0x60000000u, 4), fld(a.pending_field),
                 (int)rtm_hip::step_is_lean(total < nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, a, s16),
                 fld(a.hold_field), idx(a.snap_in2, 0x10000000u, kInt * 4), idx(a.snap16_in2, 0x20000000u, kInt * 2),
-                idx(a.snap16_max2, 0x60000000u, 4));
+                idx(a.snap16_max2, 0x60000000u, 4), fld(ad.p_next));
         }
     }
     return 0;

@@ -5,7 +5,10 @@ restating its formulas:
 
 * every step reads, as p^n and p^{n-1}, the buffers that the two previous
   steps of its phase wrote, reads the zero field exactly where the phase has
-  no such step, and writes a buffer it does not read;
+  no such step, and writes either a buffer it does not read or, in place,
+  its own p^{n-1} (the two field buffers alternate; only a held p^{n-1}
+  sends the step to the other one); a dry step (the autotuner's timing
+  launches on live jobs) writes only a scratch field, never the live state;
 * paired imaging: every snapshot slot is imaged exactly once, in descending
   order; a paired step images the held slot first, and the held buffer then
   still holds the field of that slot's backward step (no step overwrote it);
@@ -66,14 +69,21 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
     assert len(stored) == nsnap
 
     # ---- fields: what each buffer holds, followed through both phases
-    # (buffers 0-2 rotate, 3-4 are the hold buffers, 5 is the zero field)
+    # (buffers 0-1 alternate, 3-4 are the hold buffers, 5 is the zero field)
     ZERO = 5
     content = {k: None for k in range(5)}
     for phase, recs in (("F", fwd), ("B", bwd)):
         for s, r in enumerate(recs):
             assert r["zc"] == (s == 0) and r["zp"] == (s <= 1)
             assert (r["cur"] == ZERO) == (s == 0) and (r["prev"] == ZERO) == (s <= 1)
-            assert r["next"] not in (ZERO, r["prev"], r["cur"], r["hold"])
+            assert r["next"] not in (ZERO, r["cur"], r["hold"], 2)
+            # a dry (timing) step writes only the scratch field 2
+            assert r["dry_next"] == 2
+            # in place over p^{n-1} whenever p^{n-1} is a field buffer
+            if s >= 2 and r["prev"] in (0, 1):
+                assert r["next"] in (r["prev"], 3, 4)
+            else:
+                assert r["next"] != r["prev"]
             if s >= 1:
                 assert content[r["cur"]] == (phase, s - 1)
             if s >= 2:
