@@ -307,9 +307,12 @@ typedef struct {
     double    rebuild_seconds;   /* launch-table rebuilds (inside run_seconds)      */
     double    capture_seconds;   /* hipGraph capture and instantiation (inside run) */
     long long graphs_captured;   /* captured runs of ticks, one per launch pattern  */
-    long long step_launches;     /* batched step launches (one per tick and group)  */
+    long long step_launches;     /* step launches (one per tick and group; one per tick
+                                    and job on per-job streams)                    */
     long long lean_launches;     /* of which ran the lean build (no snapshot work)  */
     int       merged_launch;     /* 1: the groups present run in one launch per tick */
+    long long concurrent_ticks;  /* ticks run with each job on the per-shot kernels on a
+                                    stream of its own (2-3 jobs, one filling the chip) */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

@@ -1049,6 +1049,73 @@ int run_solo(rtm_hip_batch* b, int n) {
     return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
 }
 
+// Two or three jobs, the largest of which fills the chip on its own: each
+// job runs on the per-shot step kernels on a stream of its own, with its
+// step arguments built on the host, so the smaller jobs' launches fill the
+// large one's launch gaps instead of the large job paying the batch kernel's
+// argument reads on every tick.  (Two chip-filling jobs side by side run
+// slower than one after the other, since their fields no longer fit the
+// caches; keeping them apart is the admission's business.)  The same tile
+// function runs either way, so the results are the same bits.
+constexpr double kConcurrentCells = 1.0e6;
+
+bool run_concurrently(const rtm_hip_batch* b) {
+    if (b->merged_on || b->active.size() < 2 || b->active.size() > (size_t)kGroupStreams) return false;
+    double largest = 0.0;
+    for (const Job* j : b->active) largest = std::max(largest, (double)j->model->nxe * (double)j->model->nze);
+    return largest >= kConcurrentCells;
+}
+
+int run_concurrent(rtm_hip_batch* b, int n) {
+    const size_t nj = b->active.size();
+    auto stream_of = [&](size_t k) { return k == 0 ? b->stream : b->group_streams[k - 1]; };
+    struct PerJob {
+        const Job* job;
+        rtm_hip::StepGeometry geom;
+        int tile_x;
+        bool s16;
+        int done0;
+    };
+    std::vector<PerJob> per;
+    for (const Job* j : b->active) {
+        const Model& m = *j->model;
+        const int key = group_key(m);
+        const Group& g = b->groups[key];
+        rtm_hip::StepGeometry geom = rtm_hip::step_geometry(g.family, m.half, g.tile_x, m.nxe, m.nze, m.ldz);
+        geom.blocks_per_cu = g.blocks_per_cu;
+        geom.lds_bytes = g.lds_bytes;
+        per.push_back({j, geom, g.tile_x, key_snap16(key), j->desc.steps - j->steps_left});
+    }
+    hipError_t e = hipEventRecord(b->fork_ev, b->stream);
+    for (size_t k = 1; k < nj && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->group_streams[k - 1], b->fork_ev, 0);
+    if (e != hipSuccess) return b->hip_fail(e, "job stream fork");
+    // tick by tick, so the streams advance together
+    for (int i = 0; i < n; ++i)
+        for (size_t k = 0; k < nj; ++k) {
+            const PerJob& p = per[k];
+            const Model& m = *p.job->model;
+            const int done = p.done0 + i;
+            if (p.s16 && done == m.nt) {
+                e = rtm_hip::launch_bookkeeping(b->d_jobs, (int)k, m.half, p.tile_x, p.job->desc.ntiles, stream_of(k));
+            } else {
+                const rtm_hip::StepArgs a = p.s16 ? rtm_hip::job_args<true>(p.job->desc, done, 0)
+                                                  : rtm_hip::job_args<false>(p.job->desc, done, 0);
+                const rtm_hip::Dir dir = done < m.nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward;
+                e = rtm_hip::launch_step(dir, m.half, p.geom, a, stream_of(k));
+                b->stats.step_launches += 1;
+                b->stats.lean_launches += rtm_hip::step_is_lean(dir, a, p.s16) ? 1 : 0;
+            }
+            if (e != hipSuccess) return b->hip_fail(e, "step launch");
+        }
+    for (size_t k = 1; k < nj && e == hipSuccess; ++k) {
+        e = hipEventRecord(b->join_ev[k - 1], b->group_streams[k - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join_ev[k - 1], 0);
+    }
+    if (e != hipSuccess) return b->hip_fail(e, "job stream join");
+    e = rtm_hip::launch_batch_advance(b->d_jobs, (int)nj, n, b->stream);
+    return e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "clock launch");
+}
+
 // Step launches of a run of ticks, and how many ran the lean build (a
 // merged launch counts as lean when every job's build is).
 void count_launches(rtm_hip_batch* b, int ticks, int ahead) {
@@ -1081,6 +1148,9 @@ bool launch_bound(const rtm_hip_batch* b) {
 int run_ticks(rtm_hip_batch* b, int n) {
     if (b->active.size() == 1) {
         if (int rc = run_solo(b, n)) return rc;
+    } else if (run_concurrently(b)) {
+        if (int rc = run_concurrent(b, n)) return rc;
+        b->stats.concurrent_ticks += n;
     } else if (b->cfg.use_graph && launch_bound(b)) {
         for (int ahead = 0; ahead < n;) {
             const int ticks = n - ahead >= kGraphTicks ? kGraphTicks : 1;

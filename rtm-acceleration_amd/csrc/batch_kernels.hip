@@ -50,6 +50,18 @@ __global__ __launch_bounds__(64) void k_tick_args(const JobDesc* jobs, int njobs
     T.build = 4 * J.half + (s16 ? 2 : 0) + (lean ? 1 : 0);
 }
 
+// The bookkeeping tick of job jid on its own (a job running on the per-shot
+// step kernels): one wave per tile of its tile_x-column tiles.
+template <int H>
+__global__ __launch_bounds__(kBlockThreads) void k_bookkeeping(const JobDesc* jobs, int jid, int tile_x) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
+    const int lane = (int)threadIdx.x & (kTileZ - 1);
+    const JobDesc& J = jobs[jid];
+    const int t = (int)blockIdx.x * kWaves + wave;
+    if (t >= J.ntiles) return;
+    bookkeeping_tile<H>(J, t, lane, tile_x);
+}
+
 // Job clocks: every job has done `ticks` more steps.
 __global__ __launch_bounds__(64) void k_advance(JobDesc* jobs, int njobs, int ticks) {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -75,6 +87,21 @@ hipError_t launch_batch_step(int half, Family family, int tile_x, bool snap16, c
                                           lean)
                   : launch_batch_step_f32(half, family, tile_x, jobs, targs, tile_job, ntiles, sub, lds_bytes, stream,
                                           lean);
+}
+
+hipError_t launch_bookkeeping(const JobDesc* jobs, int jid, int half, int tile_x, int ntiles, hipStream_t stream) {
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((ntiles + kWaves - 1) / kWaves)), block(kBlockThreads);
+    switch (half) {
+        case 1: hipLaunchKernelGGL(k_bookkeeping<1>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        case 2: hipLaunchKernelGGL(k_bookkeeping<2>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        case 3: hipLaunchKernelGGL(k_bookkeeping<3>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        case 4: hipLaunchKernelGGL(k_bookkeeping<4>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        case 5: hipLaunchKernelGGL(k_bookkeeping<5>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        case 6: hipLaunchKernelGGL(k_bookkeeping<6>, grid, block, 0, stream, jobs, jid, tile_x); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_tick_args(const JobDesc* jobs, int njobs, TickArgs* targs, int nsub, int dry,

@@ -125,4 +125,8 @@ hipError_t launch_stack_add(const float* image, const float* illum, int nx, int 
 // Job clocks: every job has done `ticks` more steps.
 hipError_t launch_batch_advance(JobDesc* jobs, int njobs, int ticks, hipStream_t stream);
 
+// The bookkeeping tick of 16-bit job jobs[jid] alone, over its ntiles
+// tile_x-column tiles (a job that runs on the per-shot step kernels).
+hipError_t launch_bookkeeping(const JobDesc* jobs, int jid, int half, int tile_x, int ntiles, hipStream_t stream);
+
 }  // namespace rtm_hip

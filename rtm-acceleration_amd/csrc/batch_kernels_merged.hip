@@ -19,7 +19,7 @@ using namespace device;
 template <int H, int TX, bool S16, bool LEAN>
 __device__ __forceinline__ void merged_tile(const JobDesc& J, const StepArgs& a, int kind, int t, int lane) {
     if (S16 && kind == kTickBookkeeping) {
-        if (!a.dry) bookkeeping_tile<H, TX>(J, t, lane);
+        if (!a.dry) bookkeeping_tile<H>(J, t, lane, TX);
         return;
     }
     if (kind == kTickForward) march_tile<H, TX, Dir::Forward, true, S16, LEAN>(a, a.coef, t, lane);

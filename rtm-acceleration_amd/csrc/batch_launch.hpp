@@ -15,13 +15,13 @@ namespace {
 using namespace device;
 
 // The bookkeeping tick of a 16-bit job, between its phases (wave t of the
-// job's TX-column tiles): the stored forward steps among the last two have
+// job's tx_cols-column tiles): the stored forward steps among the last two have
 // no step it+2 to settle their scale.  Step nt-2: its codes, from its field
 // (still in its buffer), with the max of its slices; step nt-1: the slices of
 // its per-tile maxima, which backward step 0 reduces to the max it images
 // with.
-template <int H, int TX>
-__device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int lane) {
+template <int H>
+__device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int lane, int tx_cols) {
     constexpr int TZ = kTileZ - 2 * H;
     const int st = J.store, nt = J.nt;
     if (nt >= 2 && ((nt - 2) % st) == 0) {
@@ -35,9 +35,8 @@ __device__ __forceinline__ void bookkeeping_tile(const JobDesc& J, int t, int la
         const int z = tz * TZ + lane;
         const int jz = z - J.nb;
         const bool z_int = lane >= H && lane < kTileZ - H && z < J.nze - H && jz >= 0 && jz < J.nz;
-#pragma unroll
-        for (int r = 0; r < TX; ++r) {
-            const int x = H + tx * TX + r, jx = x - J.nb;
+        for (int r = 0; r < tx_cols; ++r) {
+            const int x = H + tx * tx_cols + r, jx = x - J.nb;
             if (z_int && x < J.nxe - H && jx >= 0 && jx < J.nx)
                 __builtin_nontemporal_store(encode_snapshot(gbl(f)[(ptrdiff_t)x * J.ldz + z], k, J.snap_fmt),
                                             gbl(out) + (ptrdiff_t)jx * J.nz + jz);
@@ -71,7 +70,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restri
     const int t = g - T.tile_first;
     const StepArgs& a = T.a;
     if (S16 && kind == kTickBookkeeping) {
-        if (!a.dry) bookkeeping_tile<H, TX>(jobs[jid], t, lane);
+        if (!a.dry) bookkeeping_tile<H>(jobs[jid], t, lane, TX);
         return;
     }
     if constexpr (F == Family::kStream) {

@@ -80,7 +80,8 @@ class BatchStats(C.Structure):
                 ("group_waves_per_cu", C.c_int * 12), ("group_snap16", C.c_int * 12), ("tunes", C.c_int),
                 ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double),
                 ("graphs_captured", C.c_longlong), ("step_launches", C.c_longlong),
-                ("lean_launches", C.c_longlong), ("merged_launch", C.c_int)]
+                ("lean_launches", C.c_longlong), ("merged_launch", C.c_int),
+                ("concurrent_ticks", C.c_longlong)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

@@ -65,10 +65,15 @@ class ModelSpec:
         return 2.0 * self.nt * self.cells_per_tick
 
 
-# Admission limit of serve(): about the extended grid of one 2301 x 751 job
-# (2421 x 871 cells), the size at which one job's step already spreads over
-# every CU with several waves each.
-FILL_CELLS = 2.2e6
+# Admission limits of serve().  FILL_CELLS extended cells per tick: one
+# 2301 x 751 job (2421 x 871 cells, already enough to spread over every CU)
+# with a 1201 x 401 and a 201 x 151 job beside it, but never two 2301 x 751
+# jobs, which run slower side by side than one after the other (their fields
+# no longer fit the caches).  MAX_CONCURRENT jobs at once: the batch engine
+# runs up to three jobs of which one fills the chip each on its own stream,
+# where the smaller ones fill the large one's launch gaps.
+FILL_CELLS = 3.0e6
+MAX_CONCURRENT = 3
 
 
 def served_models(nsources: int = 4, snapshot_format: int = SNAP_F16) -> dict[str, ModelSpec]:
@@ -125,12 +130,13 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
 
     Admission is first come first served with a fill limit: an arrived
     request joins the running batch while the batch's extended cells per
-    tick, its own included, stay within fill_cells, or when nothing runs;
-    otherwise it waits, and those behind it too.  One 2301 x 751 job already
-    fills the chip, so large jobs run one after the other (processor sharing
-    among them would delay every one of them to the end of the busy period),
-    while small ones share ticks.  fill_cells <= 0 admits every request on
-    arrival.  Returns a summary."""
+    tick, its own included, stay within fill_cells and fewer than
+    MAX_CONCURRENT requests run, or when nothing runs; otherwise it waits,
+    and those behind it too.  One 2301 x 751 job already fills the chip, so
+    large jobs run one after the other (processor sharing among them would
+    delay every one of them to the end of the busy period), while smaller
+    ones run beside them.  fill_cells <= 0 admits every request on arrival.
+    Returns a summary."""
     with BatchEngine(device, memory_budget=memory_budget, use_graph=use_graph) as b:
         mids = {k: b.add_model(sp.model, sp.params, sp.nt, sp.dt, sp.snapshot_format) for k, sp in specs.items()}
         if reserve:
@@ -151,7 +157,7 @@ def serve(specs: dict[str, ModelSpec], requests: list[Request], device: int = 0,
                 r = requests[waiting[0]]
                 cells = specs[r.key].cells_per_tick
                 busy = sum(c for _, _, c in out.values())
-                if out and fill_cells > 0 and busy + cells > fill_cells:
+                if out and fill_cells > 0 and (busy + cells > fill_cells or len(out) >= MAX_CONCURRENT):
                     break
                 jid = b.submit_shot(mids[r.key], r.shot)
                 out[jid] = (waiting.pop(0), (time.perf_counter() - t0) - r.arrival, cells)

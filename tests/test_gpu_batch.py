@@ -115,6 +115,37 @@ def test_batch_kernel_families(jobs, alone, family, tile):
             assert bits_equal(alone[k][1], S), f"job {k}"
 
 
+@pytest.mark.parametrize("max_ticks", [0, 37])
+def test_concurrent_streams_equal_alone(hip_engine_cls, max_ticks):
+    """Two or three jobs, one of which fills the chip (>= 1 M extended cells),
+    run each on the per-shot kernels on a stream of its own: the results are
+    the plain engine's bits, for FP32, FP16 (codes of step nt-2 in the
+    bookkeeping tick) and BF16 (slices of step nt-1) jobs; with max_ticks the
+    runs stop and resume mid-way (one call's ticks left queued)."""
+    import rtm_amd
+    print(f"case: concurrent per-job streams, max_ticks={max_ticks}", flush=True)
+    specs = [(1601, 601, 8, 20, 300, 4, 0, 0),      # 1641 x 641 extended cells
+             (61, 41, 8, 20, 100, 2, 1, 0),        # (nt - 2) % store == 0
+             (52, 47, 12, 16, 121, 4, 2, 3)]       # (nt - 1) % store == 0
+    jobs = [_job(20 + k, *sp) for k, sp in enumerate(specs)]
+    alone = [_alone(hip_engine_cls, *j) for j in jobs]
+    with rtm_amd.BatchEngine(0) as b:
+        ids = [b.submit(m, p, nt, dt, s, snapshot_format=f) for (m, p, nt, dt, s, f) in jobs]
+        if max_ticks:
+            while b.stats()["jobs_done"] < len(jobs):
+                b.run(max_ticks=max_ticks, until_done=False)
+        else:
+            b.run()
+        st = b.stats()
+        # the small jobs finish while the large one runs: concurrent ticks
+        # until the second job finishes, then the large job alone
+        assert st["concurrent_ticks"] >= 2 * 121 + 1
+        for k, jid in enumerate(ids):
+            I, S = b.result(jid)
+            assert bits_equal(alone[k][0], I), f"job {k}: image max diff {np.abs(alone[k][0] - I).max()}"
+            assert bits_equal(alone[k][1], S), f"job {k}: illumination differs"
+
+
 def test_interleaved_submission_and_budget(jobs, alone):
     """Jobs submitted while others run, under a budget that holds two jobs."""
     import rtm_amd
