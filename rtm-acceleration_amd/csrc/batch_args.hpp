@@ -54,8 +54,10 @@ __host__ __device__ inline StepArgs job_args(const JobDesc& J, int total, int dr
         a.p_next = bwd_out(J, s);
     }
     // a dry step (the autotuner's timing launches on live jobs) must not
-    // overwrite the job's p^{n-1} in place, nor a held field
-    if (dry) a.p_next = J.fld[2];
+    // overwrite the job's p^{n-1} in place, nor a held field: it updates the
+    // scratch field in place instead, which touches as many arrays as a real
+    // step (its values are never used)
+    if (dry) a.p_prev = a.p_next = J.fld[2];
     a.coef = J.coef;
     a.vdt2 = J.vdt2;
     a.wx = J.wx;

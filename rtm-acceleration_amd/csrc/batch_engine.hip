@@ -521,14 +521,22 @@ int rebuild(rtm_hip_batch* b) {
     return rc;
 }
 
+bool run_concurrently(const rtm_hip_batch* b);
+
+// Whether the active jobs run on the per-shot step kernels (one job, or two
+// or three on streams of their own) rather than on the batch kernel.
+bool per_shot_kernels(const rtm_hip_batch* b) { return b->active.size() == 1 || run_concurrently(b); }
+
 // The tile table of one launch over the active jobs idx: the tuning of the
 // launch's load class (when it has been tuned before), every job's entry
-// tables at the launch's tile width, and tile_job uploaded.
+// tables at the launch's tile width, and tile_job uploaded.  The class is
+// the octave of the launch's extended cells and whether its jobs run on the
+// per-shot kernels, which favour other tiles than the batch kernel.
 int build_group_table(rtm_hip_batch* b, Group& g, const std::vector<int>& idx) {
     if (b->cfg.kernel_family == 0) {
         double cells = 0;
         for (int k : idx) cells += (double)b->active[k]->model->nxe * b->active[k]->model->nze;
-        const int cls = load_class_of(cells);
+        const int cls = 2 * load_class_of(cells) + (per_shot_kernels(b) ? 1 : 0);
         if (cls != g.load_class) {
             g.load_class = cls;
             auto it = g.by_load.find(cls);
@@ -728,8 +736,68 @@ int autotune(rtm_hip_batch* b) {
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? RTM_HIP_OK : b->hip_fail(e, "autotune events");
+    // A group whose jobs run on the per-shot step kernels (run_solo,
+    // run_concurrent) is timed on those, launched as those paths launch
+    // them: every job of the group takes dry steps of its next eight steps
+    // (skipping a 16-bit job's bookkeeping tick) per repetition, so the
+    // timing sees the loop's mix of lean, storing and imaging steps.
+    const bool per_shot = per_shot_kernels(b);
+    auto time_per_shot = [&](int key, const Group& g, int rounds, int reps) -> float {
+        struct Launch {
+            rtm_hip::StepArgs a;
+            rtm_hip::Dir dir;
+            int half;
+            rtm_hip::StepGeometry geom;
+        };
+        std::vector<Launch> launches;
+        for (const Job* j : b->active) {
+            if (group_key(*j->model) != key) continue;
+            const Model& m = *j->model;
+            const bool s16 = key_snap16(key);
+            const int done = j->desc.steps - j->steps_left;
+            for (int step = done; step < std::min(done + 8, j->desc.steps); ++step) {
+                if (s16 && step == m.nt) continue;
+                Launch l;
+                l.a = s16 ? rtm_hip::job_args<true>(j->desc, step, 1) : rtm_hip::job_args<false>(j->desc, step, 1);
+                l.dir = step < m.nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward;
+                l.half = m.half;
+                l.geom = rtm_hip::step_geometry(g.family, m.half, g.tile_x, m.nxe, m.nze, m.ldz);
+                l.geom.blocks_per_cu = g.blocks_per_cu;
+                l.geom.lds_bytes = g.lds_bytes;
+                launches.push_back(l);
+            }
+        }
+        // a per-shot launch is short: rounds of at least 48 launches, as the
+        // plain engine's autotuner times them, resolve a 2 % difference
+        const int n = std::max<int>(reps, (48 + (int)launches.size() - 1) / std::max<int>((int)launches.size(), 1));
+        float ms = 1e30f;
+        for (int round = 0; round < rounds && !rc; ++round) {
+            for (int rep = 0; rep < (round == 0 ? n + 2 : n) && !rc; ++rep) {
+                if (rep == (round == 0 ? 2 : 0)) (void)hipEventRecord(t0, b->stream);
+                for (const Launch& l : launches) {
+                    hipError_t le = rtm_hip::launch_step(l.dir, l.half, l.geom, l.a, b->stream);
+                    if (le != hipSuccess) {
+                        rc = b->hip_fail(le, "autotune launch");
+                        break;
+                    }
+                }
+            }
+            if (rc) break;
+            (void)hipEventRecord(t1, b->stream);
+            hipError_t se = hipEventSynchronize(t1);
+            if (se != hipSuccess) {
+                rc = b->hip_fail(se, "autotune");
+                break;
+            }
+            float round_ms = 0.0f;
+            (void)hipEventElapsedTime(&round_ms, t0, t1);
+            ms = std::min(ms, round_ms);
+        }
+        return ms;
+    };
     // 2 warm dry steps, then the best of `rounds` rounds of `reps` timed dry steps
     auto time_group = [&](int key, const Group& g, int rounds, int reps) -> float {
+        if (per_shot) return time_per_shot(key, g, rounds, reps);
         float ms = 1e30f;
         // dry tick arguments for tick 0 of the current clocks (the next run
         // rewrites the table)
@@ -769,15 +837,22 @@ int autotune(rtm_hip_batch* b) {
         // stage 1: mapping and tile width at full occupancy
         float best = 1e30f;
         Tuning pick = g;
+        // (per-shot kernels: the DPP march widths only, with no occupancy
+        // stage, which is where the plain engine's search always ends on
+        // the grids a job runs alone or beside smaller ones)
         std::vector<rtm_hip::Family> families{rtm_hip::Family::kMarchDpp};
-        if (full) families = {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch};
+        if (full && !per_shot)
+            families = {rtm_hip::Family::kMarchDpp, rtm_hip::Family::kStream, rtm_hip::Family::kMarch};
         for (auto fam : families) {
-            // widest first: with many jobs per launch there are waves to spare
-            // and wide tiles re-read less halo (a narrower one must win by 2 %)
+            // batch kernel: widest first, since with many jobs per launch there
+            // are waves to spare and wide tiles re-read less halo (a narrower
+            // one must win by 2 %); per-shot kernels: the plain engine's order
+            // of preference (4 columns first on the DPP march)
             std::vector<int> widths = rtm_hip::tile_widths(fam);
-            std::sort(widths.begin(), widths.end(), [](int x, int y) {
-                return (x == 16) != (y == 16) ? y == 16 : x > y;      // 16 last: it spills registers
-            });
+            if (!per_shot)
+                std::sort(widths.begin(), widths.end(), [](int x, int y) {
+                    return (x == 16) != (y == 16) ? y == 16 : x > y;  // 16 last: it spills registers
+                });
             for (int tx : widths) {
                 if (rc) break;
                 g.tile_x = tx;
@@ -796,7 +871,7 @@ int autotune(rtm_hip_batch* b) {
         // stage 2: wavefronts per CU for the winner
         g.tile_x = pick.tile_x;
         g.family = pick.family;
-        if (full && !rc && (rc = rebuild(b)) == RTM_HIP_OK) {
+        if (full && !per_shot && !rc && (rc = rebuild(b)) == RTM_HIP_OK) {
             for (int cap : {4, 2}) {
                 g.blocks_per_cu = cap;
                 g.lds_bytes = rtm_hip::occupancy_lds_bytes(b->device, cap);

@@ -62,7 +62,8 @@ int main(int argc, char** argv) {
                 "\"red_in\":%lld,\"red_count\":%lld,\"red_out\":%lld,\"red_out_count\":%lld,"
                 "\"qparts\":%lld,\"qcount\":%lld,\"qout\":%lld,\"qmax\":%lld,"
                 "\"s16in\":%lld,\"s16max\":%lld,\"pending\":%lld,\"lean\":%d,"
-                "\"hold\":%lld,\"snap_in2\":%lld,\"s16in2\":%lld,\"s16max2\":%lld,\"dry_next\":%lld}\n",
+                "\"hold\":%lld,\"snap_in2\":%lld,\"s16in2\":%lld,\"s16max2\":%lld,\"dry_next\":%lld,"
+                "\"dry_prev\":%lld,\"dry_cur\":%lld}\n",
                 argv[c], total, total < nt ? "fwd" : "bwd", a.it, fld(a.p_prev), fld(a.p_cur), fld(a.p_next),
                 a.zero_prev, a.zero_cur, idx(a.snap_out, 0x10000000u, kInt * 4), idx(a.snap_in, 0x10000000u, kInt * 4),
                 idx(a.wave_max, 0x30000000u, kCap * 4), idx(a.wave_count, 0x50000000u, 4),
@@ -73,7 +74,7 @@ int main(int argc, char** argv) {
                 idx(a.snap16_in, 0x20000000u, kInt * 2), idx(a.snap16_max, 0x60000000u, 4), fld(a.pending_field),
                 (int)rtm_hip::step_is_lean(total < nt ? rtm_hip::Dir::Forward : rtm_hip::Dir::Backward, a, s16),
                 fld(a.hold_field), idx(a.snap_in2, 0x10000000u, kInt * 4), idx(a.snap16_in2, 0x20000000u, kInt * 2),
-                idx(a.snap16_max2, 0x60000000u, 4), fld(ad.p_next));
+                idx(a.snap16_max2, 0x60000000u, 4), fld(ad.p_next), fld(ad.p_prev), fld(ad.p_cur));
         }
     }
     return 0;

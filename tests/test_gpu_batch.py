@@ -278,10 +278,12 @@ def test_single_short_job_after_autotune(jobs, alone, use_graph):
 
 
 def test_retune_on_load_class_change(jobs, alone):
-    """A job joining a lone order-8 job doubles the group's cells (a new load
-    class): the group is re-tuned mid-stream on dry steps, and when the
-    joiner retires the first class's cached choice comes back without a
-    search; both images keep the plain engine's bits."""
+    """A small job joining a lone order-8 job moves the lone job's group from
+    the per-shot kernels to the batch kernel (a new load class): the group is
+    re-tuned mid-stream on dry steps and the joiner's own group (16-bit
+    snapshots) is tuned on first sight; when the joiner retires, the first
+    class's cached choice comes back without a search; both images keep the
+    plain engine's bits."""
     import rtm_amd
     print("case: jobs 0 and 3 (order 8), job 3 joining at tick 40", flush=True)
     with rtm_amd.BatchEngine(0) as b:
@@ -290,7 +292,7 @@ def test_retune_on_load_class_change(jobs, alone):
         assert b.stats()["tunes"] == 1
         j3 = b.submit(*jobs[3][:5], snapshot_format=jobs[3][5])
         b.run()
-        assert b.stats()["tunes"] == 2
+        assert b.stats()["tunes"] == 3
         for k, jid in ((0, j0), (3, j3)):
             I, S = b.result(jid)
             assert bits_equal(alone[k][0], I), f"job {k}"

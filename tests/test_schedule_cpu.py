@@ -8,7 +8,8 @@ restating its formulas:
   no such step, and writes either a buffer it does not read or, in place,
   its own p^{n-1} (the two field buffers alternate; only a held p^{n-1}
   sends the step to the other one); a dry step (the autotuner's timing
-  launches on live jobs) writes only a scratch field, never the live state;
+  launches on live jobs) updates only a scratch field, in place, never the
+  live state;
 * paired imaging: every snapshot slot is imaged exactly once, in descending
   order; a paired step images the held slot first, and the held buffer then
   still holds the field of that slot's backward step (no step overwrote it);
@@ -77,8 +78,9 @@ def test_schedule_dataflow(schedules, nt, store, fmt):
             assert r["zc"] == (s == 0) and r["zp"] == (s <= 1)
             assert (r["cur"] == ZERO) == (s == 0) and (r["prev"] == ZERO) == (s <= 1)
             assert r["next"] not in (ZERO, r["cur"], r["hold"], 2)
-            # a dry (timing) step writes only the scratch field 2
-            assert r["dry_next"] == 2
+            # a dry (timing) step updates only the scratch field 2, in place,
+            # and reads the live p^n
+            assert r["dry_next"] == r["dry_prev"] == 2 and r["dry_cur"] == r["cur"]
             # in place over p^{n-1} whenever p^{n-1} is a field buffer
             if s >= 2 and r["prev"] in (0, 1):
                 assert r["next"] in (r["prev"], 3, 4)
