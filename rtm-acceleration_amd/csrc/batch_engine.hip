@@ -10,7 +10,8 @@
 // private one that lives as long as the job.
 //
 // Scheduling.  Submitted jobs wait in a FIFO queue.  The head job is admitted
-// when its device footprint (three wavefields, snapshot store, image, traces,
+// when its device footprint (two wavefields updated in place, a scratch field
+// for timing steps, two held fields, snapshot store, image, traces,
 // injection tables, plus its model's tables when they are not resident yet)
 // fits in what the memory budget has left and the active-job cap allows;
 // admission allocates and uploads it and the launch tables are rebuilt.  The
@@ -21,13 +22,15 @@
 // on the device start as soon as memory frees up, and their steps overlap
 // with the running jobs' steps in the same launches (continuous batching).
 //
-// Launches.  Jobs are grouped by stencil half-width; each group has one
-// launch per tick over the concatenation of its jobs' tiles, with the tile
-// width and cross-lane method picked per half-width and load class (octave of
-// the group's extended cells) by timing dry runs on the live jobs.  A tick (the group launches plus the clock update), and a
-// run of 32 ticks (32 sets of group launches, one clock update), are captured
-// in hipGraphs and replayed until the job set changes, so one host call
-// advances every job 32 steps.
+// Launches.  Jobs are grouped by stencil half-width and snapshot width; each
+// group has one launch per tick over the concatenation of its jobs' tiles,
+// with the tile width and cross-lane method picked per group and load class
+// (octave of the group's extended cells, and which kernels run) by timing dry
+// steps on the live jobs.  Runs of up to 32 ticks (the group launches on
+// concurrent streams, one clock update) are captured in hipGraphs and
+// replayed while ticks are small enough to be launch-bound.  A job alone, or
+// two or three jobs of which one fills the chip, run on the per-shot step
+// kernels instead, each on a stream of its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -707,8 +710,8 @@ int enqueue_ticks(rtm_hip_batch* b, int ticks, int ahead) {
 
 // Kernel family, tile width and wavefronts per CU per group, picked by
 // timing dry steps (no clock advance, no image or snapshot update, p^{n+1}
-// to a buffer of its own, so the live state is untouched) on the live jobs; the launches are
-// long, so a candidate replaces the best when it is 2 % faster.
+// updated in place in a scratch field, so the live state is untouched) on the
+// live jobs; a candidate replaces the best when it is 2 % faster.
 int autotune_merged(rtm_hip_batch* b);
 
 int autotune(rtm_hip_batch* b) {
