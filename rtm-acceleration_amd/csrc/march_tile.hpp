@@ -681,13 +681,15 @@ __device__ __forceinline__ void strip_imaging_ops(const StepArgs& a, const Strip
     }
 }
 
-// p^{n+1} of output cell (x, lane) to the field, and its snapshot / imaging
-// epilogue (src/rtm/forward.cpp:112-118; src/rtm/backward.cpp:66-75).  The
-// caller has checked that the lane and the column are outputs.
-template <Dir D, bool S16>
+// p^{n+1} of output cell (x, lane) to the field (NTS: a non-temporal store),
+// and its snapshot / imaging epilogue (src/rtm/forward.cpp:112-118;
+// src/rtm/backward.cpp:66-75).  The caller has checked that the lane and the
+// column are outputs.
+template <Dir D, bool S16, bool NTS = false>
 __device__ __forceinline__ void strip_store(const StepArgs& a, const SnapCtx& sc, const Strip& s, int x, float out,
                                             const ColumnOps& o, float& vmax) {
-    field_col(a.p_next, a, s, x)[s.lz] = out;
+    if constexpr (NTS) __builtin_nontemporal_store(out, field_col(a.p_next, a, s, x) + s.lz);
+    else field_col(a.p_next, a, s, x)[s.lz] = out;
     const int jx = x - a.nb;
     if constexpr (D == Dir::Forward && S16) {
         // one lane predicate for the interior instead of early exits: the
@@ -786,12 +788,16 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     // early exit past the last column: only the stores are guarded, so no
     // branch stands between the tile's loads.
     const bool plain = e0 == e1;                            // wave-uniform
+    // Forward steps store p^{n+1} non-temporally (5.94 against 6.22 us per
+    // 2301 x 751 step); backward steps gain nothing from it, and batched
+    // backward ticks lose.
+    constexpr bool NTS = D == Dir::Forward;
     float out[TX];
 #pragma unroll
     for (int r = 0; r < TX; ++r) {
         out[r] = settle(leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv));
         const int x = x0 + r;
-        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S>(a, sc, s, x, out[r], ops[r], vmax);
+        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S, NTS>(a, sc, s, x, out[r], ops[r], vmax);
     }
     if (!plain) {
         // a tile with entries waits for its entry registers once, here
@@ -802,7 +808,7 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
         apply_tile_entries<D, TX>(a, te, cell, out, ops, e0, e1);
 #pragma unroll
         for (int r = 0; r < TX; ++r)
-            if (cell[r] >= 0) strip_store<D, S>(a, sc, s, x0 + r, out[r], ops[r], vmax);
+            if (cell[r] >= 0) strip_store<D, S, NTS>(a, sc, s, x0 + r, out[r], ops[r], vmax);
     }
     finish_tile<S>(a, sc, vmax, lane, t, a.ntiles);
     // the reduction of the previous stored step's maxima runs in lean
