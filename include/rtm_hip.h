@@ -293,7 +293,8 @@ typedef struct {
     long long rebuilds;          /* launch-table rebuilds (job set changes)         */
     double    cell_steps;        /* 2*nt*nxe*nze summed over finished jobs          */
     double    run_seconds;       /* wall time inside rtm_hip_batch_run, of which: */
-    double    admit_seconds;     /*   admission (allocation, tables, upload)       */
+    double    admit_seconds;     /*   admission (allocation, tables, upload), also
+                                    ahead of a change while the device runs ticks */
     double    tune_seconds;      /*   kernel autotuning                            */
     double    retire_seconds;    /*   image download, stack update, freeing        */
     double    busy_latency_s;    /* sum of submit-to-done times of finished jobs   */
@@ -313,6 +314,8 @@ typedef struct {
     int       merged_launch;     /* 1: the groups present run in one launch per tick */
     long long concurrent_ticks;  /* ticks run with each job on the per-shot kernels on a
                                     stream of its own (2-3 jobs, one filling the chip) */
+    long long staged_admissions; /* jobs admitted ahead, while the ticks before the job
+                                    set's next change ran (counted queued until then) */
 } rtm_hip_batch_stats;
 
 typedef struct rtm_hip_batch rtm_hip_batch;

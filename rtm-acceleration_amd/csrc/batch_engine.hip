@@ -21,6 +21,9 @@
 // memory and admits from the queue again: jobs that arrive while others are
 // on the device start as soon as memory frees up, and their steps overlap
 // with the running jobs' steps in the same launches (continuous batching).
+// When a job is about to finish and the queue head fits in the memory free
+// now, the head is admitted ahead, while the device runs the ticks up to
+// that finish; it joins the active set at the change.
 //
 // Launches.  Jobs are grouped by stencil half-width and snapshot width; each
 // group has one launch per tick over the concatenation of its jobs' tiles,
@@ -161,6 +164,9 @@ struct rtm_hip_batch {
     std::map<long long, std::unique_ptr<Job>> jobs;
     std::deque<Job*> queue;
     std::vector<Job*> active;
+    // jobs admitted ahead (stage_ahead): device state ready, joining the
+    // active set at its next change, before anything still in the queue
+    std::vector<Job*> staged;
     JobDesc* d_jobs = nullptr;
     rtm_hip::TickArgs* d_targs = nullptr;            // [jobs_cap][kRunTicks] (launch_tick_args)
     size_t jobs_cap = 0;
@@ -958,6 +964,18 @@ int autotune_merged(rtm_hip_batch* b) {
 }
 
 int admit_from_queue(rtm_hip_batch* b) {
+    // jobs admitted ahead join first (they were ahead in the queue)
+    size_t joined = 0;
+    for (; joined < b->staged.size(); ++joined) {
+        if (b->cfg.max_active > 0 && (int)b->active.size() >= b->cfg.max_active) break;
+        Job* j = b->staged[joined];
+        j->state = RTM_HIP_JOB_RUNNING;
+        j->t_admit = now_s();
+        b->active.push_back(j);
+        b->dirty = true;
+    }
+    b->staged.erase(b->staged.begin(), b->staged.begin() + (ptrdiff_t)joined);
+    if (!b->staged.empty()) return RTM_HIP_OK;
     while (!b->queue.empty()) {
         Job* j = b->queue.front();
         if (b->cfg.max_active > 0 && (int)b->active.size() >= b->cfg.max_active) break;
@@ -987,6 +1005,40 @@ int admit_from_queue(rtm_hip_batch* b) {
         b->active.push_back(j);
         b->dirty = true;
     }
+    return RTM_HIP_OK;
+}
+
+// Admission ahead, called with the ticks up to the next job's finish just
+// enqueued: queue-head jobs that will join the active set at that change
+// (within the active-job cap, counting the finishing jobs out) and fit in the
+// memory free now are admitted while the device runs those ticks, instead of
+// on an idle device after the change.  Their uploads and clears are
+// synchronous copies and fills on the null stream, which the batch's
+// non-blocking streams do not wait for, into memory no queued tick touches
+// (the finishing jobs keep theirs until they retire).  A job that does not
+// fit, or whose admission fails, stays queued for the ordinary admission,
+// which reports any error.
+int stage_ahead(rtm_hip_batch* b) {
+    int finishing = 0;
+    for (const Job* j : b->active) finishing += j->steps_left == 0;
+    if (finishing == 0) return RTM_HIP_OK;
+    const double t0 = now_s();
+    while (!b->queue.empty()) {
+        const int after = (int)b->active.size() - finishing + (int)b->staged.size();
+        if (b->cfg.max_active > 0 && after >= b->cfg.max_active) break;
+        Job* j = b->queue.front();
+        if (b->used + admission_bytes(*j) > b->budget) break;
+        if (admit_job(b, *j) != RTM_HIP_OK) {
+            job_free(b, *j);
+            b->err.clear();
+            break;
+        }
+        b->queue.pop_front();
+        b->used += j->bytes;
+        b->staged.push_back(j);
+        b->stats.staged_admissions += 1;
+    }
+    b->stats.admit_seconds += now_s() - t0;
     return RTM_HIP_OK;
 }
 
@@ -1340,6 +1392,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     (void)hipStreamSynchronize(b->stream);
     b->drop_graph();
     for (Job* j : b->active) job_free(b, *j);
+    for (Job* j : b->staged) job_free(b, *j);
     for (auto& [id, j] : b->jobs) model_release(b, *j->model);
     for (auto& [id, m] : b->models) model_release(b, *m);
     for (auto& [key, g] : b->groups)
@@ -1366,7 +1419,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
 
 int rtm_hip_batch_reserve(rtm_hip_batch* b, size_t bytes) {
     if (int rc = check(b)) return rc;
-    if (!b->active.empty() || !b->queue.empty())
+    if (!b->active.empty() || !b->queue.empty() || !b->staged.empty())
         return b->fail(RTM_HIP_ERR_ARG, "reserve memory before submitting jobs");
     if (b->arena.active()) return b->fail(RTM_HIP_ERR_ARG, "memory is already reserved");
     if (bytes == 0) bytes = b->budget > b->used ? b->budget - b->used : 0;
@@ -1480,6 +1533,7 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
         if (n <= 0) break;
         if ((rc = run_ticks(b, n))) break;
         ran += n;
+        if ((rc = stage_ahead(b))) break;
         const double tr = now_s();
         rc = retire_finished(b);
         b->stats.retire_seconds += now_s() - tr;
@@ -1533,6 +1587,8 @@ int rtm_hip_batch_set_stack(rtm_hip_batch* b, int nx, int nz) {
     if (nx <= 0 || nz <= 0) return b->fail(RTM_HIP_ERR_ARG, "stack size must be positive");
     for (Job* j : b->queue)
         if (j->stack_ix0 >= 0) return b->fail(RTM_HIP_ERR_ARG, "queued jobs use the current stack");
+    for (Job* j : b->staged)
+        if (j->stack_ix0 >= 0) return b->fail(RTM_HIP_ERR_ARG, "queued jobs use the current stack");
     for (Job* j : b->active)
         if (j->stack_ix0 >= 0) return b->fail(RTM_HIP_ERR_ARG, "active jobs use the current stack");
     (void)hipSetDevice(b->device);
@@ -1573,7 +1629,7 @@ int rtm_hip_batch_get_stats(rtm_hip_batch* b, rtm_hip_batch_stats* out) {
     if (!out) return RTM_HIP_OK;
     *out = b->stats;
     out->jobs_active = (int)b->active.size();
-    out->jobs_queued = (int)b->queue.size();
+    out->jobs_queued = (int)(b->queue.size() + b->staged.size());
     out->models_resident = (int)b->models.size();
     out->bytes_in_use = b->used;
     out->bytes_budget = b->budget;

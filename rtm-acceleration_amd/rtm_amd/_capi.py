@@ -81,7 +81,7 @@ class BatchStats(C.Structure):
                 ("rebuild_seconds", C.c_double), ("capture_seconds", C.c_double),
                 ("graphs_captured", C.c_longlong), ("step_launches", C.c_longlong),
                 ("lean_launches", C.c_longlong), ("merged_launch", C.c_int),
-                ("concurrent_ticks", C.c_longlong)]
+                ("concurrent_ticks", C.c_longlong), ("staged_admissions", C.c_longlong)]
 
 
 # Every entry point declared in include/rtm_hip.h, with its ctypes signature.

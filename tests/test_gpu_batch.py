@@ -90,6 +90,9 @@ def test_batch_equals_alone(jobs, alone, use_graph, max_active, merge):
             assert st["ticks"] > longest
         if max_active:
             assert st["rebuilds"] > 2      # jobs joined mid-run
+            # the jobs that waited were admitted ahead, while the ticks up to
+            # the finish that let them in ran
+            assert st["staged_admissions"] == len(jobs) - max_active
         if merge:
             assert st["merged_launch"] in (0, 1) and all(g["tile_x"] in (2, 3, 4) for g in st["groups"])
         for k, jid in enumerate(ids):
