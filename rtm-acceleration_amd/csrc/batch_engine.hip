@@ -102,6 +102,7 @@ struct Job {
     std::vector<void*> entry_allocs; // injection tables of the current tile width
     JobDesc desc{};
     int tile_x_built = 0;            // tile width the injection tables were built for
+                                     // (negative: entry-free tables for timing, stub_job_entries)
     int steps_left = 0;              // ticks until done (host mirror of the device clock)
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::vector<float> image, illum;
@@ -169,6 +170,11 @@ struct rtm_hip_batch {
     std::vector<Job*> staged;
     JobDesc* d_jobs = nullptr;
     rtm_hip::TickArgs* d_targs = nullptr;            // [jobs_cap][kRunTicks] (launch_tick_args)
+    // while the autotuner times candidates, the launch tables give every job
+    // entry-free tile tables (stub_job_entries) in this table of zeros
+    bool timing = false;
+    int* d_zero_ptr = nullptr;
+    size_t zero_cap = 0;
     size_t jobs_cap = 0;
     std::map<int, Group> groups;                     // by group_key (half-width, snapshot width)
     // One launch per tick for every job (cfg.merge_groups, with jobs of more
@@ -402,12 +408,17 @@ void job_cell(const Model& m, float x, float z, int* ixe, int* ize) {
 }
 
 // Per-tile injection tables of the job for tile width tile_x (forward: the
-// source; backward: every receiver, in receiver order within a cell).
+// source; backward: every receiver, in receiver order within a cell), both
+// directions in one block uploaded with one copy: tile pointers, entries,
+// tile pointers, entries, each part at an even int offset (8-byte aligned
+// entries).
 int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
     free_entries(b, j);
     const Model& m = *j.model;
     const rtm_hip::StepGeometry g =
         rtm_hip::step_geometry(rtm_hip::Family::kMarchDpp, m.half, tile_x, m.nxe, m.nze, m.ldz);
+    std::vector<int> blob;
+    size_t off_ptr[2], off_ent[2];
     for (int k = 0; k < 2; ++k) {
         struct T {
             int tile, cell, code;
@@ -428,33 +439,66 @@ int build_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
             if (a.cell != c.cell) return a.cell < c.cell;
             return a.code < c.code;
         });
-        std::vector<int> ptr((size_t)g.ntiles + 1, 0);
-        std::vector<Entry> ent(v.size());
-        for (size_t i = 0; i < v.size(); ++i) {
-            ent[i] = Entry{v[i].cell, v[i].code};
-            ++ptr[(size_t)v[i].tile + 1];
-        }
+        off_ptr[k] = blob.size();
+        blob.resize(off_ptr[k] + (size_t)g.ntiles + 1, 0);
+        int* ptr = blob.data() + off_ptr[k];
+        for (const T& t : v) ++ptr[(size_t)t.tile + 1];
         for (int t = 0; t < g.ntiles; ++t) ptr[t + 1] += ptr[t];
-        int* d_ptr = nullptr;
-        Entry* d_ent = nullptr;
-        if (int rc = job_alloc(b, j.entry_allocs, &d_ptr, ptr.size(), "injection tables")) return rc;
-        if (int rc = job_alloc(b, j.entry_allocs, &d_ent, ent.size(), "injection tables")) return rc;
-        hipError_t e = hipMemcpy(d_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice);
-        if (e == hipSuccess && !ent.empty())
-            e = hipMemcpy(d_ent, ent.data(), ent.size() * sizeof(Entry), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return b->hip_fail(e, "injection table upload");
-        j.desc.tile_ptr[k] = d_ptr;
-        j.desc.entries[k] = d_ent;
+        if (blob.size() & 1) blob.push_back(0);
+        off_ent[k] = blob.size();
+        for (const T& t : v) {
+            blob.push_back(t.cell);
+            blob.push_back(t.code);
+        }
+    }
+    int* d = nullptr;
+    if (int rc = job_alloc(b, j.entry_allocs, &d, blob.size(), "injection tables")) return rc;
+    hipError_t e = hipMemcpy(d, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return b->hip_fail(e, "injection table upload");
+    for (int k = 0; k < 2; ++k) {
+        j.desc.tile_ptr[k] = d + off_ptr[k];
+        j.desc.entries[k] = reinterpret_cast<const Entry*>(d + off_ent[k]);
     }
     // the receiver amplitudes in the new backward-entry order
-    hipError_t e = rtm_hip::launch_gather_amps(j.desc.entries[1], j.desc.amp_count, j.d_traces, j.trace_nt, m.nt,
-                                               j.desc.amps, b->stream);
+    e = rtm_hip::launch_gather_amps(j.desc.entries[1], j.desc.amp_count, j.d_traces, j.trace_nt, m.nt, j.desc.amps,
+                                    b->stream);
     if (e != hipSuccess) return b->hip_fail(e, "amplitude table");
     j.desc.ntz = g.ntz;
     j.desc.ntz_magic = g.ntz_magic;
     j.desc.ntx = g.ntx;
     j.desc.ntiles = g.ntiles;
     j.tile_x_built = tile_x;
+    return RTM_HIP_OK;
+}
+
+// The tile geometry of width tile_x with no injection entries, for the
+// autotuner's dry steps, which inject and record nothing: every tile's entry
+// range is empty in the handle's shared table of zeros.  The job's own tables
+// are rebuilt for the width the search picks.
+int stub_job_entries(rtm_hip_batch* b, Job& j, int tile_x) {
+    free_entries(b, j);
+    const Model& m = *j.model;
+    const rtm_hip::StepGeometry g =
+        rtm_hip::step_geometry(rtm_hip::Family::kMarchDpp, m.half, tile_x, m.nxe, m.nze, m.ldz);
+    const size_t need = (size_t)g.ntiles + 1;
+    if (need > b->zero_cap) {
+        if (b->d_zero_ptr) (void)hipFree(b->d_zero_ptr);
+        b->d_zero_ptr = nullptr;
+        b->zero_cap = 0;
+        hipError_t e = hipMalloc(&b->d_zero_ptr, need * sizeof(int));
+        if (e == hipSuccess) e = hipMemset(b->d_zero_ptr, 0, need * sizeof(int));
+        if (e != hipSuccess) return b->hip_fail(e, "empty tile table");
+        b->zero_cap = need;
+    }
+    for (int k = 0; k < 2; ++k) {
+        j.desc.tile_ptr[k] = b->d_zero_ptr;
+        j.desc.entries[k] = reinterpret_cast<const Entry*>(b->d_zero_ptr);
+    }
+    j.desc.ntz = g.ntz;
+    j.desc.ntz_magic = g.ntz_magic;
+    j.desc.ntx = g.ntx;
+    j.desc.ntiles = g.ntiles;
+    j.tile_x_built = -tile_x;
     return RTM_HIP_OK;
 }
 
@@ -556,8 +600,11 @@ int build_group_table(rtm_hip_batch* b, Group& g, const std::vector<int>& idx) {
     std::vector<int> tile_job;
     for (int k : idx) {
         Job& j = *b->active[k];
-        if (j.tile_x_built != g.tile_x)
+        if (b->timing && j.tile_x_built != -g.tile_x) {
+            if (int rc = stub_job_entries(b, j, g.tile_x)) return rc;
+        } else if (!b->timing && j.tile_x_built != g.tile_x) {
             if (int rc = build_job_entries(b, j, g.tile_x)) return rc;
+        }
         j.desc.tile_first = (int)tile_job.size();
         tile_job.insert(tile_job.end(), (size_t)j.desc.ntiles, k);
     }
@@ -1400,6 +1447,7 @@ void rtm_hip_batch_destroy(rtm_hip_batch* b) {
     if (b->merged.d_tile_job) (void)hipFree(b->merged.d_tile_job);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_targs) (void)hipFree(b->d_targs);
+    if (b->d_zero_ptr) (void)hipFree(b->d_zero_ptr);
     if (b->d_stack_i) (void)hipFree(b->d_stack_i);
     if (b->d_stack_s) (void)hipFree(b->d_stack_s);
     if (b->arena.active()) (void)hipFree(b->arena.base());
@@ -1524,7 +1572,11 @@ int rtm_hip_batch_run(rtm_hip_batch* b, int max_ticks, int until_done) {
             n = std::min(n, warm);
         } else {
             const double tt = now_s();
+            b->timing = true;
             rc = autotune(b);
+            b->timing = false;
+            // the searched jobs' own tables are rebuilt for the width picked
+            for (const Job* j : b->active) b->dirty |= j->tile_x_built < 0;
             b->stats.tune_seconds += now_s() - tt;
             if (rc) break;
             if (b->dirty && (rc = rebuild(b))) break;
