@@ -798,7 +798,7 @@ int autotune(rtm_hip_batch* b) {
     // (skipping a 16-bit job's bookkeeping tick) per repetition, so the
     // timing sees the loop's mix of lean, storing and imaging steps.
     const bool per_shot = per_shot_kernels(b);
-    auto time_per_shot = [&](int key, const Group& g, int rounds, int reps) -> float {
+    auto time_per_shot = [&](int key, const Group& g, int rounds, int reps, float cutoff) -> float {
         struct Launch {
             rtm_hip::StepArgs a;
             rtm_hip::Dir dir;
@@ -848,12 +848,15 @@ int autotune(rtm_hip_batch* b) {
             float round_ms = 0.0f;
             (void)hipEventElapsedTime(&round_ms, t0, t1);
             ms = std::min(ms, round_ms);
+            if (ms > cutoff) break;
         }
         return ms;
     };
     // 2 warm dry steps, then the best of `rounds` rounds of `reps` timed dry steps
-    auto time_group = [&](int key, const Group& g, int rounds, int reps) -> float {
-        if (per_shot) return time_per_shot(key, g, rounds, reps);
+    // A candidate whose best round so far is above `cutoff` (well behind the
+    // best candidate) cannot win: its remaining rounds are skipped.
+    auto time_group = [&](int key, const Group& g, int rounds, int reps, float cutoff) -> float {
+        if (per_shot) return time_per_shot(key, g, rounds, reps, cutoff);
         float ms = 1e30f;
         // dry tick arguments for tick 0 of the current clocks (the next run
         // rewrites the table)
@@ -879,6 +882,7 @@ int autotune(rtm_hip_batch* b) {
             float round_ms = 0.0f;
             (void)hipEventElapsedTime(&round_ms, t0, t1);
             ms = std::min(ms, round_ms);
+            if (ms > cutoff) break;
         }
         return ms;
     };
@@ -916,7 +920,7 @@ int autotune(rtm_hip_batch* b) {
                 g.lds_bytes = 0;
                 g.blocks_per_cu = 0;
                 if ((rc = rebuild(b))) break;
-                const float ms = time_group(key, g, rounds, reps);
+                const float ms = time_group(key, g, rounds, reps, 1.25f * best);
                 if (rc) break;
                 if (ms < 0.98f * best) {
                     best = ms;
@@ -932,7 +936,7 @@ int autotune(rtm_hip_batch* b) {
                 g.blocks_per_cu = cap;
                 g.lds_bytes = rtm_hip::occupancy_lds_bytes(b->device, cap);
                 if (g.lds_bytes == 0) continue;
-                const float ms = time_group(key, g, rounds, reps);
+                const float ms = time_group(key, g, rounds, reps, 1.25f * best);
                 if (rc) break;
                 if (ms < 0.98f * best) {
                     best = ms;
