@@ -779,15 +779,20 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     snap_scales<S>(a, sc, lane);
     float vmax = 0.0f;
 
-    // One pass over the columns computes every output once.  A tile without
-    // injection entries (nearly every tile) stores each cell as soon as it is
-    // computed, which keeps the stores streaming; a tile with entries keeps
-    // its outputs, applies the entries in one pass, then stores.  (Two
-    // separate column loops, one per case, made the compiler hoist their
-    // common sums above the branch and hold every column live at once.)  No
-    // early exit past the last column: only the stores are guarded, so no
-    // branch stands between the tile's loads.
+    // One pass over the columns computes every output once and stores each
+    // cell as soon as it is computed, which keeps the stores streaming.  A
+    // tile with injection entries applies them in one pass after the
+    // columns: a lean build (a field store and nothing else per cell) has
+    // stored every column already and stores again, from the same lane to
+    // the same address, only the cells whose bits the entries changed; a
+    // build with a snapshot or imaging epilogue, which must see the value
+    // after injection, keeps its outputs and stores them after the pass.
+    // (Two separate column loops, one per case, made the compiler hoist
+    // their common sums above the branch and hold every column live at
+    // once.)  No early exit past the last column: only the stores are
+    // guarded, so no branch stands between the tile's loads.
     const bool plain = e0 == e1;                            // wave-uniform
+    const bool early = LEAN || plain;
     // Forward steps store p^{n+1} non-temporally (5.94 against 6.22 us per
     // 2301 x 751 step); backward steps gain nothing from it, and batched
     // backward ticks lose.
@@ -797,18 +802,23 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     for (int r = 0; r < TX; ++r) {
         out[r] = settle(leapfrog(q[r + H], laplacian<H, DPP>(cf, q, r + H), ops[r], wzv));
         const int x = x0 + r;
-        if (plain && x < a.nxe - H && s.z_out) strip_store<D, S, NTS>(a, sc, s, x, out[r], ops[r], vmax);
+        if (early && x < a.nxe - H && s.z_out) strip_store<D, S, NTS>(a, sc, s, x, out[r], ops[r], vmax);
     }
     if (!plain) {
         // a tile with entries waits for its entry registers once, here
         entries_ready(te);
         int cell[TX];
+        uint32_t stored[TX];
 #pragma unroll
-        for (int r = 0; r < TX; ++r) cell[r] = (s.z_out && x0 + r < a.nxe - H) ? (x0 + r) * a.ldz + s.z : -1;
+        for (int r = 0; r < TX; ++r) {
+            cell[r] = (s.z_out && x0 + r < a.nxe - H) ? (x0 + r) * a.ldz + s.z : -1;
+            stored[r] = __float_as_uint(out[r]);
+        }
         apply_tile_entries<D, TX>(a, te, cell, out, ops, e0, e1);
 #pragma unroll
         for (int r = 0; r < TX; ++r)
-            if (cell[r] >= 0) strip_store<D, S, NTS>(a, sc, s, x0 + r, out[r], ops[r], vmax);
+            if (cell[r] >= 0 && (!LEAN || __float_as_uint(out[r]) != stored[r]))
+                strip_store<D, S, NTS>(a, sc, s, x0 + r, out[r], ops[r], vmax);
     }
     finish_tile<S>(a, sc, vmax, lane, t, a.ntiles);
     // the reduction of the previous stored step's maxima runs in lean
