@@ -743,9 +743,11 @@ __device__ __forceinline__ void march_tile(const StepArgs& a, const Coeffs& cf, 
     if constexpr (!(PH & 1)) sc.store_max = false;
     if constexpr (!(PH & 2)) sc.quantise = false;
     snap_prefetch<S>(a, sc, lane);
-    // The entry bounds are scalar loads: issued before the tile's loads they
-    // are in flight with them; the 16-bit build has its slice-max load there
-    // already and does better without them.
+    // The entry bounds are scalar loads, requested here, ahead of the tile's
+    // loads (the compiler may still schedule them after those, as it does in
+    // the lean builds; either way they are waited for only before the entry
+    // loads); the 16-bit build has its slice-max load here already and does
+    // better with the bounds after the tile's loads.
     int e0 = 0, e1 = 0;
     if constexpr (!S) e0 = entry_bound(a, t), e1 = entry_bound(a, t + 1);
 
