@@ -145,6 +145,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
 // March mapping: one wave per tile (march_tile.hpp).
 template <int H, int TX, Dir D, bool DPP, bool S16, bool LEAN = false, int PH = 3>
 __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
+    request_args(a);
     // the wave index is wave-uniform: the tile's coordinates and column
     // offsets are then scalar arithmetic
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTileZ);
