@@ -37,6 +37,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_merged(const JobDesc* _
     if (g >= ntiles) return;
     const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
     const TickArgs& T = targs[jid * kRunTicks + sub];
+    // the tick's arguments in one batch with its kind and build (request_args)
+    asm volatile("" ::"s"(T.kind), "s"(T.tile_first), "s"(T.build));
+    request_args(T.a);
     const int kind = T.kind;
     if (kind == kTickIdle) return;           // finished job: its tiles idle
     const int t = g - T.tile_first;

@@ -65,6 +65,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch(const JobDesc* __restri
     if (g >= ntiles) return;
     const int jid = __builtin_amdgcn_readfirstlane(tile_job[g]);
     const TickArgs& T = targs[jid * kRunTicks + sub];
+    // the tick's arguments in one batch with its kind (request_args)
+    asm volatile("" ::"s"(T.kind), "s"(T.tile_first));
+    request_args(T.a);
     const int kind = T.kind;
     if (kind == kTickIdle) return;           // finished job: its tiles idle
     const int t = g - T.tile_first;
