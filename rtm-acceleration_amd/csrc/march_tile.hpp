@@ -39,16 +39,18 @@ __device__ __forceinline__ T sld(const T* p) {
 // one wait.  Left to itself, it requests them in two or three rounds (around
 // the tile-index arithmetic and the early exit, and some after the field
 // loads), each round a scalar round trip on the wave's critical path
-// (backward 2301 x 751 step 7.66 -> 7.46 us, forward 5.89 -> 5.80 us).  More
-// operands than these make it reload some of them after the wait.
-// (The stencil coefficients of shifts 1-4 are among them; an order-10 or
-// order-12 stencil loads its last ones later.)
+// (backward 2301 x 751 step 7.66 -> 7.46 us, forward 5.89 -> 5.78 us).  More
+// operands than these make it reload some of them after the wait; the
+// entry count and the time step, first needed by the entry loads after the
+// field loads, come with the entry bounds there.  (The stencil coefficients
+// of shifts 1-4 are among them; an order-10 or order-12 stencil loads its
+// last ones later.)
 __device__ __forceinline__ void request_args(const StepArgs& a) {
     asm volatile("" ::"s"(a.p_cur), "s"(a.p_prev), "s"(a.vdt2), "s"(a.wz), "s"(a.wx), "s"(a.p_next),
                  "s"(a.ldz), "s"(a.nze), "s"(a.nxe), "s"(a.ntz_magic), "s"(a.ntz), "s"(a.ntiles));
-    asm volatile("" ::"s"(a.tile_ptr), "s"(a.entries), "s"(a.amps), "s"(a.amp_count), "s"(a.it),
-                 "s"(a.coef.c0), "s"(a.coef.cx[1]), "s"(a.coef.cx[2]), "s"(a.coef.cx[3]), "s"(a.coef.cx[4]),
-                 "s"(a.coef.cz[1]), "s"(a.coef.cz[2]), "s"(a.coef.cz[3]), "s"(a.coef.cz[4]));
+    asm volatile("" ::"s"(a.tile_ptr), "s"(a.entries), "s"(a.amps), "s"(a.coef.c0), "s"(a.coef.cx[1]),
+                 "s"(a.coef.cx[2]), "s"(a.coef.cx[3]), "s"(a.coef.cx[4]), "s"(a.coef.cz[1]), "s"(a.coef.cz[2]),
+                 "s"(a.coef.cz[3]), "s"(a.coef.cz[4]));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Give XCD k the
