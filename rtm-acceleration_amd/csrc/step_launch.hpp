@@ -26,6 +26,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_step(StepArgs a) {
     constexpr int LX = TX + 2 * H;           // LDS rows (columns + halo)
     __shared__ float tile[LX * LZ];
 
+    request_args(a);
     const int t = xcd_tile((int)blockIdx.x, a.ntiles);
     const int tz = t % a.ntz, tx = t / a.ntz;
     const int z0 = tz * kTileZ, x0 = tx * TX;
@@ -160,6 +161,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_march(StepArgs a) {
 // Stream mapping: one wave per strip of XS columns (march_tile.hpp).
 template <int H, int XS, Dir D, bool S16>
 __global__ __launch_bounds__(kBlockThreads) void k_stream(StepArgs a) {
+    request_args(a);
     const int wave = (int)threadIdx.x / kTileZ;
     const int lane = (int)threadIdx.x & (kTileZ - 1);
     const int nblocks = (a.ntiles + kWaves - 1) / kWaves;
